@@ -1,0 +1,112 @@
+/*
+ * zoda_hip.h -- C ABI of the MI355X (gfx950) ZODA column encoder.
+ *
+ * All arithmetic is exact over BabyBear, p = 15 * 2^27 + 1 = 2013265921.
+ * Values cross this boundary as canonical u32 in [0, p).  Every entry point
+ * returns an int status: 0 on success, a negative ZODA_E* code for an argument
+ * or resource error, or a positive hipError_t from the HIP runtime.  No entry
+ * point aborts the process; the message of the last failure on the calling
+ * thread is available from zoda_hip_last_error().
+ *
+ * Reference interfaces replaced (jonas089/zoda-cu):
+ *   src/ntt/cuda.rs:15-22          extern "C" block of the Rust crate
+ *   cuda/ntt_kernel.cu:184-208     cuda_host_alloc / cuda_host_free /
+ *                                  cuda_ntt / cuda_intt
+ *   src/benchmarks/utils.rs:100-146 encode_gpu (transpose + pad + INTT + NTT)
+ *   src/ntt/cpu.rs:97-110          roots_of_unity (host side of run(),
+ *                                  src/ntt/cuda.rs:112)
+ */
+#ifndef ZODA_HIP_H
+#define ZODA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZODA_P 2013265921u
+
+#define ZODA_OK 0
+#define ZODA_EINVAL (-1)  /* bad argument: n not a power of two, stride < n, ... */
+#define ZODA_ENOMEM (-2)  /* device or pinned host allocation failed */
+#define ZODA_ENODEV (-3)  /* no HIP device visible */
+#define ZODA_EBUDGET (-4) /* request larger than the configured memory budget */
+#define ZODA_ESTATE (-5)  /* object used in the wrong state */
+
+/* Output layouts of the encode entry points. */
+#define ZODA_LAYOUT_ROW 0 /* out[r * cols + c]: the extended square, row-major */
+#define ZODA_LAYOUT_COL 1 /* out[c * n_kn + r]: PinnedSquare layout, src/ntt/cuda.rs:31-67 */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* zoda_hip_last_error(void);
+
+/* ---- drop-in for the reference FFI --------------------------------------- */
+
+/* Replaces the direct cudaGetDeviceCount binding, src/ntt/cuda.rs:21,25-29.
+ * Writes the number of visible HIP devices (0 when none) and returns 0. */
+int zoda_hip_device_count(int* count);
+
+/* Replaces cuda_host_alloc, cuda/ntt_kernel.cu:184-187: `count` u32 of pinned
+ * host memory, NULL on failure (and for count == 0). */
+uint32_t* zoda_hip_host_alloc(size_t count);
+
+/* Replaces cuda_host_free, cuda/ntt_kernel.cu:188.  NULL is a no-op. */
+void zoda_hip_host_free(uint32_t* p);
+
+/* Replaces cuda_ntt, cuda/ntt_kernel.cu:190-193 / src/ntt/cuda.rs:17.
+ * `values` is host memory (pinned or pageable) holding `polys` polynomials
+ * `stride` u32 apart; the first `n` values of each are replaced by their
+ * natural-order NTT, X[k] = sum_j x_j roots[(j*k) mod n]; values n..stride-1
+ * of each polynomial are left untouched.  `roots` holds n canonical values
+ * roots[i] = w^i for a primitive n-th root of unity w (as produced by
+ * zoda_hip_roots_of_unity); the transform uses exactly the roots given.
+ * Synchronous. */
+int zoda_hip_ntt(uint32_t* values, const uint32_t* roots, uint32_t n, uint32_t polys,
+                 uint32_t stride);
+
+/* Replaces cuda_intt, cuda/ntt_kernel.cu:196-208 / src/ntt/cuda.rs:18.  Same
+ * contract; computes x_j = n^-1 sum_k X_k roots[(-j*k) mod n]. */
+int zoda_hip_intt(uint32_t* values, const uint32_t* roots, uint32_t n, uint32_t polys,
+                  uint32_t stride);
+
+/* The n roots of unity of src/ntt/cpu.rs:97-110: w = 31^((p-1)/n) (31 is the
+ * smallest generator of F_p^*), roots[i] = w^i.  n must be a power of two
+ * dividing p - 1 (n <= 2^27). */
+int zoda_hip_roots_of_unity(uint32_t n, uint32_t* roots);
+
+/* ---- device-resident transforms and encode -------------------------------- */
+
+/* Batched NTT (inverse = 0) or INTT (inverse = 1) on device memory, same
+ * layout and result as zoda_hip_ntt / zoda_hip_intt with the standard roots.
+ * Asynchronous on `stream` (a hipStream_t, NULL for the null stream). */
+int zoda_hip_ntt_device(uint32_t* d_values, uint32_t n, uint32_t polys, uint32_t stride,
+                        int inverse, void* stream);
+
+/* ZODA column encode of encode_gpu (src/benchmarks/utils.rs:100-146) on device
+ * memory.  d_in is k rows of `cols` canonical values, row-major.  Each column
+ * is zero-padded to n_k = next_pow2(k), inverse-transformed at size n_k,
+ * zero-padded to n_kn = next_pow2(k + n) and transformed at size n_kn.  d_out
+ * receives n_kn x cols values in `layout`.  d_in and d_out must not overlap.
+ * Asynchronous on `stream`. */
+int zoda_hip_encode_device(const uint32_t* d_in, uint32_t k, uint32_t cols, uint32_t n,
+                           uint32_t* d_out, int layout, void* stream);
+
+/* The same encode end to end from host memory: h_in row-major k x cols, h_out
+ * n_kn x cols in `layout`.  Column slabs are streamed through the device with
+ * copies overlapping the kernels; no host-side transpose.  Pinned buffers
+ * (zoda_hip_host_alloc) give full PCIe bandwidth.  Synchronous. */
+int zoda_hip_encode_host(const uint32_t* h_in, uint32_t k, uint32_t cols, uint32_t n,
+                         uint32_t* h_out, int layout);
+
+/* Fills d_out[i] for i < count with the seeded uniform BabyBear generator:
+ * for attempt a = 0, 1, ...: z = splitmix64(seed + (64*i + a + 1) * 0x9E3779B97F4A7C15),
+ * v = z >> 33; the first v < p is the value.  Asynchronous on `stream`. */
+int zoda_hip_fill_uniform(uint32_t* d_out, size_t count, uint64_t seed, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZODA_HIP_H */

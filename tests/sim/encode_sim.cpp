@@ -1,0 +1,43 @@
+// CPU emulation of the fused encode kernel -- TEST INFRASTRUCTURE ONLY.
+//
+// Runs the exact per-thread code of zoda-cu_amd/csrc/encode_core.hpp on the
+// host: for every tile, each step is executed for every thread id before the
+// next step starts (the work-group barrier), with LDS modelled by a vector.
+// The tests compare its output with the oracle to check the kernel's index and
+// twiddle arithmetic on the CPU, before and besides the GPU parity tests.  The
+// product library does not contain or call this code.
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../zoda-cu_amd/csrc/encode_plans.hpp"
+#include "../../zoda-cu_amd/csrc/encode_tables.hpp"
+
+using namespace zoda;
+
+template <class PL, int STEP>
+static void run_steps(uint32_t tile, const uint32_t* in, uint32_t* out, uint32_t krows,
+                      uint32_t cols, int layout, int32_t* lds, const EncodeTabs& T) {
+  for (int tid = 0; tid < PL::NT; tid++)
+    encode_step<PL, STEP>(tid, tile, in, out, krows, cols, layout, lds, T);
+  if constexpr (STEP + 1 < PL::STEPS) run_steps<PL, STEP + 1>(tile, in, out, krows, cols, layout, lds, T);
+}
+
+extern "C" int zoda_sim_encode(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t n,
+                               uint32_t* out, int layout) {
+  uint32_t nk = 1, L = 0;
+  while (nk < k) { nk <<= 1; L++; }
+  if (k == 0 || cols == 0 || k + n <= nk || k + n > 2 * nk) return -1;
+  return with_encode_plan((int)L, [&](auto plan) -> int {
+    using PL = decltype(plan);
+    EncodeTabOffsets off;
+    std::vector<Tw> tabs = build_encode_tables(PL::L, PL::R1, PL::R2, PL::RD, &off);
+    EncodeTabs T{tabs.data() + off.inv1, tabs.data() + off.inv2, tabs.data() + off.fwd1,
+                 tabs.data() + off.fwd2, tabs.data() + off.twist, tabs.data() + off.roots};
+    std::vector<int32_t> lds(PL::LDS_WORDS ? PL::LDS_WORDS : 1);
+    const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
+    for (uint32_t tile = 0; tile < ntiles; tile++)
+      run_steps<PL, 0>(tile, in, out, k, cols, layout, lds.data(), T);
+    return 0;
+  });
+}

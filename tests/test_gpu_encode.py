@@ -1,0 +1,190 @@
+"""GPU parity of the fused encode (zoda_hip_encode_device / _host, replacing
+encode_gpu of src/benchmarks/utils.rs:100-146) against the oracle.
+
+Whole squares are compared bit for bit up to 1024^2.  At 2048^2 and 4096^2
+(the sizes the oracle cannot finish in seconds) the checks are
+size-independent: seeded sampled columns against the oracle, the systematic
+even rows, a random linear combination of all columns commuting with the
+extension (src/benchmarks/utils.rs:221-272), and reconstruction of erased rows
+from a random s of the 2s rows."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import lagrange_eval, rlc_rows
+
+pytestmark = pytest.mark.gpu
+P = O.P
+
+
+@pytest.fixture(scope="module")
+def zh():
+    import zoda_hip
+    assert zoda_hip.gpu_available(), "no HIP device"
+    return zoda_hip
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    return torch
+
+
+def to_dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint32).view(np.int32)).cuda()
+
+
+def to_host(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+def gpu_encode(zh, torch, data, n=None, layout=0):
+    out = zh.encode_tensor(to_dev(torch, data), n=n, layout=layout)
+    torch.cuda.synchronize()
+    return to_host(out)
+
+
+def print_case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 8, 16, 17, 32, 64, 100, 128, 256, 512, 1000, 1024, 2048, 4096])
+def test_small_widths_all_sides(zh, torch, k):
+    rng = np.random.default_rng(k)
+    cols = 37 if k <= 1024 else 13
+    data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)
+    print_case("small", k=k, cols=cols)
+    expect = O.encode(data, threads=8)
+    assert np.array_equal(gpu_encode(zh, torch, data), expect)
+    assert np.array_equal(gpu_encode(zh, torch, data, layout=1).T, expect)
+
+
+@pytest.mark.parametrize("side", [64, 128, 256, 512, 1024])
+def test_whole_square_celestia(zh, torch, side):
+    # side x side square of 512-byte shares: side rows x 128*side columns
+    cols = 128 * side
+    data = np.empty((side, cols), dtype=np.uint32)
+    t = torch.empty((side, cols), dtype=torch.int32, device="cuda")
+    zh.fill_uniform(t, seed=side)
+    torch.cuda.synchronize()
+    data[:] = to_host(t)
+    print_case("square", side=side)
+    expect = O.encode(data, threads=16)
+    got = zh.encode_tensor(t)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(got), expect)
+
+
+def test_reference_ramp_input(zh, torch):
+    # src/benchmarks/utils.rs:81-84 input on a 256^2 square, validated the
+    # way validate_zoda_encoding does (64 evenly spaced columns, all rows)
+    side = 256
+    cols = 128 * side
+    data = O.ramp_input(side, cols)
+    got = gpu_encode(zh, torch, data)
+    idx = [(i * cols) // 64 for i in range(64)]
+    assert np.array_equal(got[:, idx], O.encode_columns(data, idx))
+
+
+def test_uniform_generator_matches_host(zh, torch):
+    t = torch.empty(1 << 20, dtype=torch.int32, device="cuda")
+    zh.fill_uniform(t, seed=1234)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(t), zh.uniform_numpy(1 << 20, 1234))
+
+
+@pytest.mark.parametrize("k,n", [(5, 4), (6, 10), (100, 29), (1000, 1000), (33, 32), (8, 1), (8, 0),
+                                 (3, 20), (700, 2000), (5000, 5000)])
+def test_uneven_and_general_shapes(zh, torch, k, n):
+    # n_kn = 2 n_k runs the fused kernel; other shapes (n_kn = n_k, n_kn >= 4 n_k,
+    # n_k > 4096) run the staged transpose + batched-transform path
+    rng = np.random.default_rng(k + 7 * n)
+    cols = 21
+    data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)
+    print_case("shape", k=k, n=n)
+    expect = O.encode(data, n=n, threads=8)
+    assert np.array_equal(gpu_encode(zh, torch, data, n=n), expect)
+    assert np.array_equal(gpu_encode(zh, torch, data, n=n, layout=1).T, expect)
+
+
+def test_edge_values(zh, torch):
+    for fill in (0, 1, P - 1):
+        data = np.full((2048, 64), fill, dtype=np.uint32)
+        assert np.array_equal(gpu_encode(zh, torch, data), O.encode(data, threads=8))
+
+
+def test_host_end_to_end(zh, torch):
+    side = 256
+    cols = 128 * side
+    rng = np.random.default_rng(3)
+    data = rng.integers(0, P, size=(side, cols), dtype=np.uint32)
+    expect = O.encode(data, threads=16)
+    assert np.array_equal(zh.encode_host(data), expect)
+    assert np.array_equal(zh.encode_host(data, layout=1).T, expect)
+    # pinned input and output buffers
+    pin_in = zh.PinnedSquare(side * cols, 1)
+    pin_out = zh.PinnedSquare(2 * side * cols, 1)
+    pin_in.array[:] = data.ravel()
+    zh.encode_host(pin_in.array.reshape(side, cols), out=pin_out.array.reshape(2 * side, cols))
+    assert np.array_equal(pin_out.array.reshape(2 * side, cols), expect)
+
+
+def big_square_checks(zh, torch, side, seed):
+    cols = 128 * side
+    t_in = torch.empty((side, cols), dtype=torch.int32, device="cuda")
+    zh.fill_uniform(t_in, seed=seed)
+    print_case("big-encode", side=side)
+    t_out = zh.encode_tensor(t_in)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(seed)
+    # 1. sampled columns, bit-exact against the oracle
+    cols_idx = sorted(set(rng.choice(cols, size=24, replace=False).tolist()) | {0, cols - 1})
+    idx = torch.tensor(cols_idx, device="cuda")
+    data_s = to_host(t_in[:, idx].contiguous())
+    got_s = to_host(t_out[:, idx].contiguous())
+    print_case("big-sampled", side=side, ncols=len(cols_idx))
+    assert np.array_equal(got_s, O.encode(data_s, threads=16))
+    # 2. systematic even rows equal the data everywhere
+    assert torch.equal(t_out[0::2], t_in)
+    # 3. RLC: sum_c coeff_c * out[:, c] is the extension of sum_c coeff_c * in[:, c]
+    coeffs = rng.integers(0, P, size=(cols, 4), dtype=np.uint32)
+    c_dev = torch.from_numpy(coeffs.astype(np.int64)).cuda()
+
+    def rlc_dev(rows_t):
+        acc = torch.zeros((rows_t.shape[0], 4), dtype=torch.int64, device="cuda")
+        for c0 in range(0, cols, 1 << 16):
+            blk = rows_t[:, c0:c0 + (1 << 16)].to(torch.int64) & 0xFFFFFFFF
+            for limb in range(4):
+                prod = (blk * c_dev[c0:c0 + (1 << 16), limb]) % P
+                acc[:, limb] = (acc[:, limb] + prod.sum(dim=1)) % P
+        return acc.cpu().numpy().astype(np.uint32)
+
+    print_case("big-rlc", side=side)
+    rlc_in = rlc_dev(t_in)                       # side x 4
+    ext = O.encode(rlc_in)                        # 2 side x 4: the extended RLC vector
+    rows_idx = sorted(rng.choice(2 * side, size=64, replace=False).tolist())
+    rlc_out = rlc_dev(t_out[torch.tensor(rows_idx, device="cuda")])
+    assert np.array_equal(rlc_out, ext[rows_idx])
+    # cross-check the device RLC on the CPU for a few rows
+    host_rows = to_host(t_out[torch.tensor(rows_idx[:2], device="cuda")])
+    assert np.array_equal(rlc_rows(host_rows, coeffs), rlc_out[:2])
+    # 4. erasure: any s of the 2s rows give back the rest (two sampled columns)
+    print_case("big-erasure", side=side)
+    w = int(O.roots_of_unity(2 * side)[1])
+    pts = np.array([O.pow_mod(w, r) for r in range(2 * side)], dtype=np.uint64)
+    keep = np.sort(rng.choice(2 * side, size=side, replace=False))
+    lost = np.setdiff1d(np.arange(2 * side), keep)[:32]
+    for j in (0, len(cols_idx) // 2):
+        col = got_s[:, j]
+        rec = lagrange_eval(pts[keep], col[keep], pts[lost])
+        assert np.array_equal(rec, col[lost])
+    del t_in, t_out
+    torch.cuda.empty_cache()
+
+
+def test_big_square_2048(zh, torch):
+    big_square_checks(zh, torch, 2048, seed=2048)
+
+
+def test_big_square_4096(zh, torch):
+    big_square_checks(zh, torch, 4096, seed=4096)

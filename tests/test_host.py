@@ -1,0 +1,106 @@
+"""CPU-only checks of the product's host side: the C ABI library loads and
+exports every entry point include/zoda_hip.h declares, host-only entry points
+and argument validation behave, and the encode kernel's per-thread code (run
+on the CPU through tests/sim) reproduces the oracle bit for bit."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import ROOT, sim_encode
+
+P = O.P
+
+
+def _declared_functions():
+    text = open(os.path.join(ROOT, "include", "zoda_hip.h")).read()
+    return sorted(set(re.findall(r"\b(zoda_hip_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    import zoda_hip
+    names = _declared_functions()
+    assert len(names) >= 10
+    for name in names:
+        assert hasattr(zoda_hip.lib, name), name
+
+
+def test_device_count_without_gpu():
+    import ctypes
+    import zoda_hip
+    c = ctypes.c_int(-1)
+    assert zoda_hip.lib.zoda_hip_device_count(ctypes.byref(c)) == 0
+    assert c.value >= 0
+    assert zoda_hip.lib.zoda_hip_device_count(None) == -1
+
+
+def test_roots_of_unity_matches_oracle(gold):
+    import zoda_hip
+    assert list(zoda_hip.roots_of_unity(8)) == gold["roots_8"]
+    for n in (1, 2, 1024, 1 << 16):
+        assert np.array_equal(zoda_hip.roots_of_unity(n), O.roots_of_unity(n))
+    with pytest.raises(zoda_hip.ZodaError) as e:
+        zoda_hip.roots_of_unity(12)
+    assert e.value.code == -1
+
+
+def test_argument_errors_do_not_touch_the_device():
+    import zoda_hip
+    lib = zoda_hip.lib
+    buf = np.zeros(64, dtype=np.uint32)
+    roots = np.zeros(64, dtype=np.uint32)
+    # n not a power of two, stride < n, NULL pointers -> ZODA_EINVAL, no abort
+    assert lib.zoda_hip_ntt(buf.ctypes.data, roots.ctypes.data, 3, 1, 3) == -1
+    assert "power of two" in zoda_hip.last_error()
+    assert lib.zoda_hip_intt(buf.ctypes.data, roots.ctypes.data, 64, 1, 32) == -1
+    assert "stride" in zoda_hip.last_error()
+    assert lib.zoda_hip_ntt(None, roots.ctypes.data, 8, 1, 8) == -1
+    assert lib.zoda_hip_encode_device(None, 4, 4, 4, None, 0, None) == -1
+    assert lib.zoda_hip_encode_device(buf.ctypes.data, 0, 4, 4, buf.ctypes.data, 0, None) == -1
+    assert lib.zoda_hip_encode_device(buf.ctypes.data, 4, 4, 4, buf.ctypes.data, 7, None) == -1
+    assert lib.zoda_hip_encode_host(buf.ctypes.data, 4, 0, 4, buf.ctypes.data, 0) == -1
+    # the Python mirror keeps the reference's asserts (src/ntt/cuda.rs:106-108)
+    with pytest.raises(AssertionError):
+        zoda_hip.ntt_gpu(buf, 6, 6)
+    with pytest.raises(AssertionError):
+        zoda_hip.ntt_gpu(buf, 64, 32)
+    with pytest.raises(AssertionError):
+        zoda_hip.ntt_gpu(buf[:60], 8, 16)
+
+
+def test_uniform_generator_host_restatement():
+    import zoda_hip
+    a = zoda_hip.uniform_numpy(100000, 7)
+    assert a.max() < P
+    assert np.array_equal(a, zoda_hip.uniform_numpy(100000, 7))
+    assert not np.array_equal(a, zoda_hip.uniform_numpy(100000, 8))
+    # roughly uniform over [0, p)
+    hist = np.histogram(a, bins=16, range=(0, P))[0]
+    assert hist.min() > 100000 / 16 * 0.9
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 7, 8, 16, 31, 32, 33, 64, 100, 128, 256, 300, 512,
+                               1024, 2048, 4096])
+def test_kernel_emulation_matches_oracle(k):
+    rng = np.random.default_rng(k)
+    cols = 37 if k <= 1024 else 9   # ragged against every tile width
+    data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)
+    expect = O.encode(data, threads=4)
+    assert np.array_equal(sim_encode(data), expect)
+    assert np.array_equal(sim_encode(data, layout=1).T, expect)
+
+
+@pytest.mark.parametrize("k,n", [(5, 4), (6, 10), (33, 40), (100, 29)])
+def test_kernel_emulation_uneven_extension(k, n):
+    rng = np.random.default_rng(k * 100 + n)
+    data = rng.integers(0, P, size=(k, 11), dtype=np.uint32)
+    assert np.array_equal(sim_encode(data, n=n), O.encode(data, n=n))
+
+
+def test_kernel_emulation_extreme_values():
+    k, cols = 512, 40
+    for fill in (0, 1, P - 1):
+        data = np.full((k, cols), fill, dtype=np.uint32)
+        assert np.array_equal(sim_encode(data), O.encode(data, threads=4))

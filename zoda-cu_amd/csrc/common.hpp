@@ -1,0 +1,95 @@
+// Shared host-side plumbing of libzoda_hip: status codes, the per-thread last
+// error, HIP call checking and the per-device state (cached constant tables,
+// reusable device buffers and streams).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/zoda_hip.h"
+#include "field.hpp"
+#include "fp64.hpp"
+
+namespace zoda {
+
+// Records a formatted message as this thread's last error and returns `code`.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+// Evaluates a HIP call; on failure records the error and returns it (positive).
+#define ZODA_HIP(call)                                                            \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::zoda::fail((int)e_, "%s failed: %s (%s:%d)", #call,                \
+                          hipGetErrorString(e_), __FILE__, __LINE__);             \
+  } while (0)
+
+// Propagates a non-zero status.
+#define ZODA_TRY(expr)         \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ != 0) return rc_;  \
+  } while (0)
+
+inline bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+inline uint32_t log2_ceil(uint64_t x) {
+  uint32_t l = 0;
+  while ((1ull << l) < x) l++;
+  return l;
+}
+
+// Montgomery-form radix-2 twiddles of one transform size on one device:
+// tw[h + j] = w_(2h)^(+-j) * 2^32 mod p for h = 1, 2, 4, ..., n/2 and j < h.
+struct MontTables {
+  uint32_t* fwd = nullptr;
+  uint32_t* inv = nullptr;
+};
+
+struct EncodeTableSet {
+  Tw* base = nullptr;
+  size_t inv1 = 0, inv2 = 0, fwd1 = 0, fwd2 = 0, twist = 0, roots = 0;
+};
+
+struct DeviceState {
+  int device = 0;
+  std::recursive_mutex mu;
+  std::map<uint32_t, MontTables> mont;           // standard roots, keyed by log n
+  std::map<uint32_t, EncodeTableSet> encode;     // keyed by log n_k
+  // Drop-in entry points: tables built from the caller's roots.
+  std::map<uint32_t, std::vector<uint32_t>> user_roots;
+  std::map<uint32_t, MontTables> user_mont;
+  // Grow-only scratch buffers and streams for the host-memory entry points.
+  std::vector<void*> buffers;
+  std::vector<size_t> buffer_bytes;
+  std::vector<hipStream_t> streams;
+};
+
+// State of the calling thread's current device (created on first use).
+int current_state(DeviceState** out);
+int get_buffer(DeviceState* st, int slot, size_t bytes, void** out);
+int get_streams(DeviceState* st, int count, hipStream_t** out);
+
+// Standard-root Montgomery tables for size 2^log_n.
+int get_mont_tables(DeviceState* st, uint32_t log_n, MontTables* out);
+// Montgomery tables from caller roots (roots[i] = w^i, i < n).
+int get_user_mont_tables(DeviceState* st, const uint32_t* roots, uint32_t log_n, MontTables* out);
+
+// Batched poly-major transform on device memory (ntt_poly.hip).
+int launch_ntt_poly(uint32_t* d_values, uint32_t log_n, uint32_t polys, uint32_t stride,
+                    int inverse, const MontTables& tabs, hipStream_t stream);
+
+// Fused column encode on device memory (encode.hip).  in_pitch/out_pitch are
+// the row pitches (elements) of the row-major input and row-major output; for
+// the column layout out_pitch is ignored.
+int launch_encode(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols,
+                  uint32_t n, uint32_t* d_out, int layout, hipStream_t stream);
+
+// Seeded uniform generator (fill.hip).
+int launch_fill_uniform(uint32_t* d_out, size_t count, uint64_t seed, hipStream_t stream);
+
+}  // namespace zoda

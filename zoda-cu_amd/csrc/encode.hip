@@ -1,0 +1,149 @@
+// Fused ZODA column encode for gfx950: one launch turns the row-major k x cols
+// input into the 2*n_k x cols extended square (see encode_core.hpp for the
+// algorithm and encode_plans.hpp for the per-side tiling).
+#include "common.hpp"
+#include "encode_plans.hpp"
+#include "encode_tables.hpp"
+
+namespace zoda {
+
+// Work-groups are handed to the eight XCDs round-robin (block b goes to XCD
+// b % 8).  Give each XCD a contiguous run of tiles, in dispatch order, so that
+// neighbouring tiles -- which share the 128-byte lines of every row -- run at
+// the same time behind the same L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+  const uint32_t x = b & 7u, i = b >> 3;
+  const uint32_t q = nb >> 3, r = nb & 7u;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+template <class PL>
+__global__ __launch_bounds__(PL::NT) void encode_kernel(const uint32_t* __restrict__ in,
+                                                        uint32_t* __restrict__ out,
+                                                        uint32_t krows, uint32_t cols,
+                                                        int layout, EncodeTabs T) {
+  extern __shared__ int32_t lds[];
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tid = (int)threadIdx.x;
+  encode_step<PL, 0>(tid, tile, in, out, krows, cols, layout, lds, T);
+  if constexpr (PL::STEPS > 1) {
+    __syncthreads();
+    encode_step<PL, 1>(tid, tile, in, out, krows, cols, layout, lds, T);
+    __syncthreads();
+    encode_step<PL, 2>(tid, tile, in, out, krows, cols, layout, lds, T);
+  }
+  if constexpr (PL::STEPS > 3) {
+    __syncthreads();
+    encode_step<PL, 3>(tid, tile, in, out, krows, cols, layout, lds, T);
+    __syncthreads();
+    encode_step<PL, 4>(tid, tile, in, out, krows, cols, layout, lds, T);
+  }
+}
+
+static int get_encode_tables(DeviceState* st, int L, int r1, int r2, int rd,
+                             const EncodeTableSet** out) {
+  std::lock_guard<std::recursive_mutex> lock(st->mu);
+  auto it = st->encode.find((uint32_t)L);
+  if (it == st->encode.end()) {
+    EncodeTabOffsets off;
+    std::vector<Tw> host = build_encode_tables(L, r1, r2, rd, &off);
+    EncodeTableSet set;
+    ZODA_HIP(hipMalloc((void**)&set.base, host.size() * sizeof(Tw)));
+    hipError_t e = hipMemcpy(set.base, host.data(), host.size() * sizeof(Tw), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(set.base);
+      return fail((int)e, "copying encode tables failed: %s", hipGetErrorString(e));
+    }
+    set.inv1 = off.inv1;
+    set.inv2 = off.inv2;
+    set.fwd1 = off.fwd1;
+    set.fwd2 = off.fwd2;
+    set.twist = off.twist;
+    set.roots = off.roots;
+    it = st->encode.emplace((uint32_t)L, set).first;
+  }
+  *out = &it->second;
+  return 0;
+}
+
+template <class PL>
+static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols,
+                       uint32_t* d_out, int layout, hipStream_t stream) {
+  const EncodeTableSet* set = nullptr;
+  ZODA_TRY(get_encode_tables(st, PL::L, PL::R1, PL::R2, PL::RD, &set));
+  EncodeTabs T{set->base + set->inv1, set->base + set->inv2, set->base + set->fwd1,
+               set->base + set->fwd2, set->base + set->twist, set->base + set->roots};
+  const size_t lds_bytes = (size_t)PL::LDS_WORDS * sizeof(int32_t);
+  if (lds_bytes > 64 * 1024)
+    ZODA_HIP(hipFuncSetAttribute((const void*)encode_kernel<PL>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+  const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
+  hipLaunchKernelGGL(encode_kernel<PL>, dim3(ntiles), dim3(PL::NT), lds_bytes, stream, d_in, d_out,
+                     k, cols, layout, T);
+  ZODA_HIP(hipGetLastError());
+  return 0;
+}
+
+// Column-major staging used for shapes the fused kernel does not cover
+// (n_kn != 2 * n_k, or n_k > 2^12): transpose with zero padding, two batched
+// transforms, transpose back.
+__global__ void gather_columns(const uint32_t* __restrict__ in, uint32_t k, uint32_t cols,
+                               uint32_t n_kn, uint32_t* __restrict__ tmp) {
+  const size_t total = (size_t)cols * n_kn;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t c = (uint32_t)(i / n_kn), r = (uint32_t)(i % n_kn);
+    tmp[i] = r < k ? in[(size_t)r * cols + c] : 0u;
+  }
+}
+
+__global__ void scatter_rows(const uint32_t* __restrict__ tmp, uint32_t cols, uint32_t n_kn,
+                             uint32_t* __restrict__ out) {
+  const size_t total = (size_t)cols * n_kn;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t r = (uint32_t)(i / cols), c = (uint32_t)(i % cols);
+    out[i] = tmp[(size_t)c * n_kn + r];
+  }
+}
+
+static int encode_general(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols,
+                          uint32_t log_nk, uint32_t log_nkn, uint32_t* d_out, int layout,
+                          hipStream_t stream) {
+  const uint32_t n_kn = 1u << log_nkn;
+  const size_t total = (size_t)cols * n_kn;
+  uint32_t* tmp = d_out;
+  if (layout == ZODA_LAYOUT_ROW) ZODA_HIP(hipMallocAsync((void**)&tmp, total * 4, stream));
+  const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 1u << 16);
+  hipLaunchKernelGGL(gather_columns, dim3(blocks), dim3(256), 0, stream, d_in, k, cols, n_kn, tmp);
+  ZODA_HIP(hipGetLastError());
+  MontTables tk, tkn;
+  int rc = get_mont_tables(st, log_nk, &tk);
+  if (!rc) rc = get_mont_tables(st, log_nkn, &tkn);
+  if (!rc) rc = launch_ntt_poly(tmp, log_nk, cols, n_kn, 1, tk, stream);
+  if (!rc) rc = launch_ntt_poly(tmp, log_nkn, cols, n_kn, 0, tkn, stream);
+  if (!rc && layout == ZODA_LAYOUT_ROW) {
+    hipLaunchKernelGGL(scatter_rows, dim3(blocks), dim3(256), 0, stream, tmp, cols, n_kn, d_out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail((int)e, "scatter_rows launch failed: %s", hipGetErrorString(e));
+  }
+  if (layout == ZODA_LAYOUT_ROW) {
+    hipError_t e = hipFreeAsync(tmp, stream);
+    if (!rc && e != hipSuccess) rc = fail((int)e, "hipFreeAsync failed: %s", hipGetErrorString(e));
+  }
+  return rc;
+}
+
+int launch_encode(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols, uint32_t n,
+                  uint32_t* d_out, int layout, hipStream_t stream) {
+  const uint32_t log_nk = log2_ceil(k);
+  const uint32_t log_nkn = log2_ceil((uint64_t)k + n);
+  if (log_nkn == log_nk + 1 && (int)log_nk <= ENCODE_MAX_LOG) {
+    return with_encode_plan((int)log_nk, [&](auto plan) -> int {
+      return launch_plan<decltype(plan)>(st, d_in, k, cols, d_out, layout, stream);
+    });
+  }
+  return encode_general(st, d_in, k, cols, log_nk, log_nkn, d_out, layout, stream);
+}
+
+}  // namespace zoda

@@ -1,0 +1,141 @@
+// Batched NTT / INTT over polynomials laid end to end (poly-major), the layout
+// of the reference's cuda_ntt / cuda_intt (cuda/ntt_kernel.cu:106-208): poly c
+// is the n values at values[c * stride]; values n..stride-1 are untouched.
+//
+// Canonical u32 values and Montgomery-form twiddles (mont_mul(x, w*2^32) =
+// x*w), radix-2 decimation in time after a bit-reversed load, exactly the
+// arithmetic of the reference's CPU ntt (src/ntt/cpu.rs:1-32) so any roots
+// the caller passes give the reference's answer.  Sizes up to 2^12 run
+// LDS-resident, one launch for all stages; larger sizes fall back to one
+// launch per stage on device memory.
+#include "common.hpp"
+
+namespace zoda {
+
+constexpr int NTT_LDS_WORDS = 4096;
+constexpr int NTT_THREADS = 256;
+
+template <bool INV>
+__global__ __launch_bounds__(NTT_THREADS) void ntt_poly_lds(uint32_t* __restrict__ data,
+                                                            uint32_t log_n, uint32_t log_ppw,
+                                                            uint32_t polys, uint32_t stride,
+                                                            const uint32_t* __restrict__ tw,
+                                                            uint32_t scale_mont) {
+  __shared__ uint32_t lds[NTT_LDS_WORDS];
+  const uint32_t n = 1u << log_n, total = n << log_ppw;
+  const size_t p0 = (size_t)blockIdx.x << log_ppw;
+  for (uint32_t idx = threadIdx.x; idx < total; idx += NTT_THREADS) {
+    const uint32_t p = idx >> log_n, j = idx & (n - 1);
+    const uint32_t r = log_n ? (__brev(j) >> (32 - log_n)) : 0u;
+    lds[(p << log_n) | r] = (p0 + p < polys) ? data[(p0 + p) * stride + j] : 0u;
+  }
+  __syncthreads();
+  for (uint32_t lh = 0; lh < log_n; lh++) {
+    const uint32_t h = 1u << lh;
+    for (uint32_t b = threadIdx.x; b < total / 2; b += NTT_THREADS) {
+      const uint32_t p = b >> (log_n - 1);
+      const uint32_t i = b & ((n >> 1) - 1);
+      const uint32_t j = i & (h - 1);
+      const uint32_t base = (p << log_n) + ((i >> lh) << (lh + 1)) + j;
+      const uint32_t u = lds[base];
+      const uint32_t v = mont_mul(lds[base + h], tw[h + j]);
+      lds[base] = add_mod(u, v);
+      lds[base + h] = sub_mod(u, v);
+    }
+    __syncthreads();
+  }
+  for (uint32_t idx = threadIdx.x; idx < total; idx += NTT_THREADS) {
+    const uint32_t p = idx >> log_n, j = idx & (n - 1);
+    if (p0 + p < polys) {
+      uint32_t v = lds[idx];
+      if (INV) v = mont_mul(v, scale_mont);
+      data[(p0 + p) * stride + j] = v;
+    }
+  }
+}
+
+__global__ void bitrev_global(uint32_t* __restrict__ data, uint32_t log_n, uint32_t polys,
+                              uint32_t stride) {
+  const size_t n = (size_t)1 << log_n, total = n * polys;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t p = t >> log_n;
+    const uint32_t i = (uint32_t)(t & (n - 1));
+    const uint32_t r = __brev(i) >> (32 - log_n);
+    if (i < r) {
+      uint32_t* poly = data + p * stride;
+      const uint32_t a = poly[i];
+      poly[i] = poly[r];
+      poly[r] = a;
+    }
+  }
+}
+
+__global__ void stage_global(uint32_t* __restrict__ data, uint32_t log_n, uint32_t lh,
+                             uint32_t polys, uint32_t stride, const uint32_t* __restrict__ tw) {
+  const size_t half_n = (size_t)1 << (log_n - 1), total = half_n * polys;
+  const uint32_t h = 1u << lh;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t p = t >> (log_n - 1);
+    const uint32_t i = (uint32_t)(t & (half_n - 1));
+    const uint32_t j = i & (h - 1);
+    uint32_t* poly = data + p * stride;
+    const uint32_t base = ((i >> lh) << (lh + 1)) + j;
+    const uint32_t u = poly[base];
+    const uint32_t v = mont_mul(poly[base + h], tw[h + j]);
+    poly[base] = add_mod(u, v);
+    poly[base + h] = sub_mod(u, v);
+  }
+}
+
+__global__ void scale_global(uint32_t* __restrict__ data, uint32_t log_n, uint32_t polys,
+                             uint32_t stride, uint32_t scale_mont) {
+  const size_t n = (size_t)1 << log_n, total = n * polys;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    uint32_t* v = data + (t >> log_n) * stride + (t & (n - 1));
+    *v = mont_mul(*v, scale_mont);
+  }
+}
+
+static unsigned grid_for(size_t work) {
+  return (unsigned)std::min<size_t>((work + NTT_THREADS - 1) / NTT_THREADS, 1u << 16);
+}
+
+int launch_ntt_poly(uint32_t* d_values, uint32_t log_n, uint32_t polys, uint32_t stride,
+                    int inverse, const MontTables& tabs, hipStream_t stream) {
+  if (polys == 0) return 0;
+  const uint32_t* tw = inverse ? tabs.inv : tabs.fwd;
+  // n^-1 = (2^-1)^log_n, as cuda_intt computes it (cuda/ntt_kernel.cu:201-203)
+  const uint32_t scale = to_mont_host(pow_mod_host((P + 1) / 2, log_n));
+  if ((1u << log_n) <= (uint32_t)NTT_LDS_WORDS) {
+    const uint32_t log_ppw = log2_ceil(NTT_LDS_WORDS) - log_n;
+    const size_t blocks = (((size_t)polys) + (1u << log_ppw) - 1) >> log_ppw;
+    if (inverse)
+      hipLaunchKernelGGL(ntt_poly_lds<true>, dim3((unsigned)blocks), dim3(NTT_THREADS), 0, stream,
+                         d_values, log_n, log_ppw, polys, stride, tw, scale);
+    else
+      hipLaunchKernelGGL(ntt_poly_lds<false>, dim3((unsigned)blocks), dim3(NTT_THREADS), 0, stream,
+                         d_values, log_n, log_ppw, polys, stride, tw, scale);
+    ZODA_HIP(hipGetLastError());
+    return 0;
+  }
+  const size_t n = (size_t)1 << log_n;
+  hipLaunchKernelGGL(bitrev_global, dim3(grid_for(n * polys)), dim3(NTT_THREADS), 0, stream,
+                     d_values, log_n, polys, stride);
+  ZODA_HIP(hipGetLastError());
+  for (uint32_t lh = 0; lh < log_n; lh++) {
+    hipLaunchKernelGGL(stage_global, dim3(grid_for(n / 2 * polys)), dim3(NTT_THREADS), 0, stream,
+                       d_values, log_n, lh, polys, stride, tw);
+    ZODA_HIP(hipGetLastError());
+  }
+  if (inverse) {
+    hipLaunchKernelGGL(scale_global, dim3(grid_for(n * polys)), dim3(NTT_THREADS), 0, stream,
+                       d_values, log_n, polys, stride, scale);
+    ZODA_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+}  // namespace zoda

@@ -1,0 +1,269 @@
+"""Python host side of the MI355X ZODA encoder (ctypes over libzoda_hip.so).
+
+Mirrors the reference crate's GPU API (src/ntt/cuda.rs, src/ntt/mod.rs) under
+the same argument meaning and error behaviour:
+
+    gpu_available()                 <- cuda_available()        src/ntt/cuda.rs:25-29
+    ntt_gpu(values, n, stride)      <- ntt_cuda                src/ntt/cuda.rs:124-126
+    intt_gpu(values, n, stride)     <- intt_cuda               src/ntt/cuda.rs:129-131
+    PinnedSquare(rows, cols)        <- PinnedSquare            src/ntt/cuda.rs:34-97
+    roots_of_unity(n)               <- roots_of_unity          src/ntt/cpu.rs:97-110
+
+plus the device-resident entry points (encode_device, ntt_device,
+fill_uniform) that take raw device pointers, and torch conveniences that pass
+a tensor's pointer and the current torch stream.  The library must be built
+(`python zoda-cu_amd/build_lib.py`); importing fails loudly when it is not.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+P = 2013265921
+LAYOUT_ROW = 0
+LAYOUT_COL = 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libzoda_hip.so")
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+
+
+class ZodaError(RuntimeError):
+    """A non-zero status from libzoda_hip (negative: argument or resource
+    error; positive: a hipError_t)."""
+
+    def __init__(self, code: int, where: str, message: str):
+        super().__init__(f"{where} failed with status {code}: {message}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libzoda_hip.so is not built ({LIB_PATH}); run zoda-cu_amd/build_lib.py")
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "zoda_hip_last_error": (ctypes.c_char_p, []),
+        "zoda_hip_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+        "zoda_hip_host_alloc": (_vp, [ctypes.c_size_t]),
+        "zoda_hip_host_free": (None, [_vp]),
+        "zoda_hip_ntt": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+        "zoda_hip_intt": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+        "zoda_hip_roots_of_unity": (ctypes.c_int, [ctypes.c_uint32, _vp]),
+        "zoda_hip_ntt_device": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                               ctypes.c_int, _vp]),
+        "zoda_hip_encode_device": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                  _vp, ctypes.c_int, _vp]),
+        "zoda_hip_encode_host": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                _vp, ctypes.c_int]),
+        "zoda_hip_fill_uniform": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_uint64, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def last_error() -> str:
+    return lib.zoda_hip_last_error().decode()
+
+
+def _check(rc: int, where: str) -> None:
+    if rc != 0:
+        raise ZodaError(rc, where, last_error())
+
+
+def _np_ptr(a: np.ndarray) -> int:
+    if a.dtype != np.uint32 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("expected a C-contiguous uint32 array")
+    return a.ctypes.data
+
+
+# ---- reference API mirror (src/ntt/cuda.rs) -------------------------------------
+
+def gpu_available() -> bool:
+    """cuda_available(): a HIP device is visible."""
+    c = ctypes.c_int(0)
+    return lib.zoda_hip_device_count(ctypes.byref(c)) == 0 and c.value > 0
+
+
+def roots_of_unity(n: int) -> np.ndarray:
+    """roots[i] = w^i, w = 31^((p-1)/n) (src/ntt/cpu.rs:97-110)."""
+    out = np.zeros(max(n, 1), dtype=np.uint32)
+    _check(lib.zoda_hip_roots_of_unity(n, _np_ptr(out)), "zoda_hip_roots_of_unity")
+    return out[:n]
+
+
+def _run(values: np.ndarray, n: int, stride: int, inverse: bool, roots=None) -> None:
+    # src/ntt/cuda.rs:99-119: the same asserts, then the FFI call
+    if n <= 0 or n & (n - 1):
+        raise AssertionError("NTT size must be power of 2")
+    if stride < n:
+        raise AssertionError("stride must be at least n")
+    if stride <= 0 or values.size % stride != 0:
+        raise AssertionError("values must be whole polynomials of stride elements")
+    polys = values.size // stride
+    if roots is None:
+        roots = roots_of_unity(n)
+    roots = np.ascontiguousarray(roots, dtype=np.uint32)
+    fn = lib.zoda_hip_intt if inverse else lib.zoda_hip_ntt
+    rc = fn(_np_ptr(values), _np_ptr(roots), n, polys, stride)
+    name = "zoda_hip_intt" if inverse else "zoda_hip_ntt"
+    if rc != 0:
+        raise ZodaError(rc, name, last_error())
+
+
+def ntt_gpu(values: np.ndarray, n: int, stride: int, roots=None) -> None:
+    """Forward NTT in place (ntt_cuda, src/ntt/cuda.rs:124-126)."""
+    _run(values, n, stride, False, roots)
+
+
+def intt_gpu(values: np.ndarray, n: int, stride: int, roots=None) -> None:
+    """Inverse NTT in place (intt_cuda, src/ntt/cuda.rs:129-131)."""
+    _run(values, n, stride, True, roots)
+
+
+class PinnedSquare:
+    """`cols` polynomials of `rows` values in pinned host memory; element
+    (row, col) at col*rows + row (src/ntt/cuda.rs:31-97).  `.array` is a
+    numpy view of the buffer (zero-filled)."""
+
+    def __init__(self, rows: int, cols: int):
+        count = max(rows * cols, 1)
+        ptr = lib.zoda_hip_host_alloc(count)
+        if not ptr:
+            raise MemoryError(f"pinned allocation of {count} u32 failed: {last_error()}")
+        self._ptr = ptr
+        self.rows, self.cols = rows, cols
+        buf = (ctypes.c_uint32 * count).from_address(ptr)
+        self._flat = np.frombuffer(buf, dtype=np.uint32, count=count)
+        self._flat[:] = 0
+        self.array = self._flat[: rows * cols]
+
+    def set(self, row: int, col: int, value: int) -> None:
+        self.array[col * self.rows + row] = value
+
+    def get(self, row: int, col: int) -> int:
+        return int(self.array[col * self.rows + row]) % P
+
+    def column(self, col: int) -> np.ndarray:
+        return self.array[col * self.rows:(col + 1) * self.rows]
+
+    def row(self, row: int) -> np.ndarray:
+        return self.array[row::self.rows].copy()
+
+    def free(self) -> None:
+        if self._ptr:
+            self.array = None
+            self._flat = None
+            lib.zoda_hip_host_free(self._ptr)
+            self._ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---- device-resident entry points (raw device pointers) -------------------------
+
+def n_kn_for(k: int, n: int) -> int:
+    r = 1
+    while r < k + n:
+        r <<= 1
+    return r
+
+
+def encode_device(d_in: int, k: int, cols: int, n: int, d_out: int, layout: int = LAYOUT_ROW,
+                  stream: int = 0) -> None:
+    _check(lib.zoda_hip_encode_device(d_in, k, cols, n, d_out, layout, stream or None),
+           "zoda_hip_encode_device")
+
+
+def ntt_device(d_values: int, n: int, polys: int, stride: int, inverse: bool = False,
+               stream: int = 0) -> None:
+    _check(lib.zoda_hip_ntt_device(d_values, n, polys, stride, 1 if inverse else 0, stream or None),
+           "zoda_hip_ntt_device")
+
+
+def fill_uniform_device(d_out: int, count: int, seed: int, stream: int = 0) -> None:
+    _check(lib.zoda_hip_fill_uniform(d_out, count, seed, stream or None), "zoda_hip_fill_uniform")
+
+
+def encode_host(data: np.ndarray, n: int | None = None, layout: int = LAYOUT_ROW,
+                out: np.ndarray | None = None) -> np.ndarray:
+    """End-to-end encode of a row-major k x cols host array (pinned or not)."""
+    if data.dtype != np.uint32 or data.ndim != 2 or not data.flags["C_CONTIGUOUS"]:
+        raise TypeError("data must be a C-contiguous 2-D uint32 array")
+    k, cols = data.shape
+    if n is None:
+        n = k
+    rows = n_kn_for(k, n)
+    shape = (rows, cols) if layout == LAYOUT_ROW else (cols, rows)
+    if out is None:
+        out = np.empty(shape, dtype=np.uint32)
+    elif out.shape != shape or out.dtype != np.uint32 or not out.flags["C_CONTIGUOUS"]:
+        raise TypeError(f"out must be a C-contiguous uint32 array of shape {shape}")
+    _check(lib.zoda_hip_encode_host(data.ctypes.data, k, cols, n, out.ctypes.data, layout),
+           "zoda_hip_encode_host")
+    return out
+
+
+# ---- torch conveniences -----------------------------------------------------------
+
+def _stream_of(t) -> int:
+    import torch
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def encode_tensor(t_in, n: int | None = None, layout: int = LAYOUT_ROW, out=None):
+    """Encode a k x cols int32/uint32 CUDA tensor (canonical values); returns
+    the n_kn x cols (or cols x n_kn) tensor, on the current torch stream."""
+    import torch
+    if t_in.dim() != 2 or not t_in.is_cuda or not t_in.is_contiguous() or t_in.element_size() != 4:
+        raise TypeError("t_in must be a contiguous 2-D 4-byte CUDA tensor")
+    k, cols = t_in.shape
+    if n is None:
+        n = k
+    rows = n_kn_for(k, n)
+    shape = (rows, cols) if layout == LAYOUT_ROW else (cols, rows)
+    if out is None:
+        out = torch.empty(shape, dtype=t_in.dtype, device=t_in.device)
+    encode_device(t_in.data_ptr(), k, cols, n, out.data_ptr(), layout, _stream_of(t_in))
+    return out
+
+
+def fill_uniform(t, seed: int) -> None:
+    """Fill a contiguous 4-byte CUDA tensor with the seeded uniform generator."""
+    if not t.is_cuda or not t.is_contiguous() or t.element_size() != 4:
+        raise TypeError("t must be a contiguous 4-byte CUDA tensor")
+    fill_uniform_device(t.data_ptr(), t.numel(), seed, _stream_of(t))
+
+
+def uniform_numpy(count: int, seed: int) -> np.ndarray:
+    """Host restatement of the device generator (zoda_hip_fill_uniform)."""
+    gamma = np.uint64(0x9E3779B97F4A7C15)
+    idx = np.arange(count, dtype=np.uint64)
+    out = np.zeros(count, dtype=np.uint32)
+    todo = np.arange(count)
+    with np.errstate(over="ignore"):
+        for a in range(64):
+            z = np.uint64(seed) + (np.uint64(64) * idx[todo] + np.uint64(a + 1)) * gamma
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+            v = (z >> np.uint64(33)).astype(np.uint32)
+            ok = v < P
+            out[todo[ok]] = v[ok]
+            todo = todo[~ok]
+            if todo.size == 0:
+                break
+    return out
