@@ -15,12 +15,13 @@
 
 using namespace zoda;
 
-template <class PL, int STEP>
+template <class PL, int STEP, bool FULL>
 static void run_steps(uint32_t tile, const uint32_t* in, uint32_t* out, uint32_t krows,
                       uint32_t cols, int layout, int32_t* lds, const EncodeTabs& T) {
   for (int tid = 0; tid < PL::NT; tid++)
-    encode_step<PL, STEP>(tid, tile, in, out, krows, cols, layout, lds, T);
-  if constexpr (STEP + 1 < PL::STEPS) run_steps<PL, STEP + 1>(tile, in, out, krows, cols, layout, lds, T);
+    encode_step<PL, STEP, FULL>(tid, tile, in, out, krows, cols, layout, lds, T);
+  if constexpr (STEP + 1 < PL::STEPS)
+    run_steps<PL, STEP + 1, FULL>(tile, in, out, krows, cols, layout, lds, T);
 }
 
 extern "C" int zoda_sim_encode(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t n,
@@ -36,8 +37,13 @@ extern "C" int zoda_sim_encode(const uint32_t* in, uint32_t k, uint32_t cols, ui
                  tabs.data() + off.fwd2, tabs.data() + off.twist, tabs.data() + off.roots};
     std::vector<int32_t> lds(PL::LDS_WORDS ? PL::LDS_WORDS : 1);
     const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
-    for (uint32_t tile = 0; tile < ntiles; tile++)
-      run_steps<PL, 0>(tile, in, out, k, cols, layout, lds.data(), T);
+    const bool full = cols % PL::W == 0 && k == (uint32_t)PL::S;
+    for (uint32_t tile = 0; tile < ntiles; tile++) {
+      if (full)
+        run_steps<PL, 0, true>(tile, in, out, k, cols, layout, lds.data(), T);
+      else
+        run_steps<PL, 0, false>(tile, in, out, k, cols, layout, lds.data(), T);
+    }
     return 0;
   });
 }

@@ -104,3 +104,12 @@ def test_kernel_emulation_extreme_values():
     for fill in (0, 1, P - 1):
         data = np.full((k, cols), fill, dtype=np.uint32)
         assert np.array_equal(sim_encode(data), O.encode(data, threads=4))
+
+
+@pytest.mark.parametrize("k,cols", [(1, 256), (16, 256), (32, 512), (64, 128), (256, 64), (512, 64),
+                                    (1024, 64), (2048, 32), (4096, 16)])
+def test_kernel_emulation_full_tiles(k, cols):
+    # whole tiles and k = 2^L select the variant compiled without bounds checks
+    rng = np.random.default_rng(k + cols)
+    data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)
+    assert np.array_equal(sim_encode(data), O.encode(data, threads=4))

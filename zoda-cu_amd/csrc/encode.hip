@@ -17,7 +17,7 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-template <class PL>
+template <class PL, bool FULL>
 __global__ __launch_bounds__(PL::NT) void encode_kernel(const uint32_t* __restrict__ in,
                                                         uint32_t* __restrict__ out,
                                                         uint32_t krows, uint32_t cols,
@@ -25,18 +25,18 @@ __global__ __launch_bounds__(PL::NT) void encode_kernel(const uint32_t* __restri
   extern __shared__ int32_t lds[];
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = (int)threadIdx.x;
-  encode_step<PL, 0>(tid, tile, in, out, krows, cols, layout, lds, T);
+  encode_step<PL, 0, FULL>(tid, tile, in, out, krows, cols, layout, lds, T);
   if constexpr (PL::STEPS > 1) {
     __syncthreads();
-    encode_step<PL, 1>(tid, tile, in, out, krows, cols, layout, lds, T);
+    encode_step<PL, 1, FULL>(tid, tile, in, out, krows, cols, layout, lds, T);
     __syncthreads();
-    encode_step<PL, 2>(tid, tile, in, out, krows, cols, layout, lds, T);
+    encode_step<PL, 2, FULL>(tid, tile, in, out, krows, cols, layout, lds, T);
   }
   if constexpr (PL::STEPS > 3) {
     __syncthreads();
-    encode_step<PL, 3>(tid, tile, in, out, krows, cols, layout, lds, T);
+    encode_step<PL, 3, FULL>(tid, tile, in, out, krows, cols, layout, lds, T);
     __syncthreads();
-    encode_step<PL, 4>(tid, tile, in, out, krows, cols, layout, lds, T);
+    encode_step<PL, 4, FULL>(tid, tile, in, out, krows, cols, layout, lds, T);
   }
 }
 
@@ -74,12 +74,17 @@ static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32
   EncodeTabs T{set->base + set->inv1, set->base + set->inv2, set->base + set->fwd1,
                set->base + set->fwd2, set->base + set->twist, set->base + set->roots};
   const size_t lds_bytes = (size_t)PL::LDS_WORDS * sizeof(int32_t);
+  const bool full = cols % PL::W == 0 && k == (uint32_t)PL::S;
+  const void* fn = full ? (const void*)encode_kernel<PL, true> : (const void*)encode_kernel<PL, false>;
   if (lds_bytes > 64 * 1024)
-    ZODA_HIP(hipFuncSetAttribute((const void*)encode_kernel<PL>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+    ZODA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
   const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
-  hipLaunchKernelGGL(encode_kernel<PL>, dim3(ntiles), dim3(PL::NT), lds_bytes, stream, d_in, d_out,
-                     k, cols, layout, T);
+  if (full)
+    hipLaunchKernelGGL((encode_kernel<PL, true>), dim3(ntiles), dim3(PL::NT), lds_bytes, stream, d_in,
+                       d_out, k, cols, layout, T);
+  else
+    hipLaunchKernelGGL((encode_kernel<PL, false>), dim3(ntiles), dim3(PL::NT), lds_bytes, stream, d_in,
+                       d_out, k, cols, layout, T);
   ZODA_HIP(hipGetLastError());
   return 0;
 }

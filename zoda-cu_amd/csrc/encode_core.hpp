@@ -46,10 +46,10 @@ namespace zoda {
 
 // Device (or host, for the emulation) pointers into one table buffer.
 struct EncodeTabs {
-  const Tw* inv1;   // INTT twiddles of the top digit, [k << LO1 | low]
-  const Tw* inv2;   // INTT twiddles of the second digit, [k << LO2 | low]
-  const Tw* fwd1;   // NTT twiddles of the top digit
-  const Tw* fwd2;   // NTT twiddles of the second digit
+  const Tw* inv1;   // INTT twiddles of the top digit, [low << R1 | k]
+  const Tw* inv2;   // INTT twiddles of the second digit, [low << R2 | k]
+  const Tw* fwd1;   // NTT twiddles of the top digit, [k << LO1 | low]
+  const Tw* fwd2;   // NTT twiddles of the second digit, [k << LO2 | low]
   const Tw* twist;  // s entries in storage order: s^-1 * w_2s^(digitrev(i))
   const Tw* roots;  // 64 entries: w_32^e for e < 32, then w_32^-e
 };
@@ -125,7 +125,7 @@ ZHD size_t out_index(uint32_t row, uint32_t cidx, uint32_t cols, uint32_t out_ro
 //   FIRST    read the tile from global memory (and write the even rows)
 //   LAST     write canonical odd rows to global memory
 //   LOA, RA  digit above, whose forward twiddle ends MID/NTT passes (RA = 0: none)
-template <class PL, int LO, int R, int KIND, bool FIRST, bool LAST, int LOA, int RA>
+template <class PL, int LO, int R, int KIND, bool FIRST, bool LAST, int LOA, int RA, bool FULL>
 ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
                      uint32_t* __restrict__ out, uint32_t krows, uint32_t cols, int layout,
                      int32_t* __restrict__ lds, const EncodeTabs& T, const Tw* __restrict__ own,
@@ -148,7 +148,7 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
 #pragma unroll
       for (int m = 0; m < RR; m++) {
         const uint32_t j = row_of<LO, R>(u, m);
-        raw[gi][m] = (cidx < cols && j < krows) ? in[(size_t)j * cols + cidx] : 0u;
+        raw[gi][m] = (FULL || (cidx < cols && j < krows)) ? in[(size_t)j * cols + cidx] : 0u;
       }
     }
 #pragma unroll
@@ -156,7 +156,7 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
       const uint32_t g = (uint32_t)(gi * PL::NT + tid);
       const uint32_t c = g % PL::W, u = g / PL::W;
       const uint32_t cidx = tile * PL::W + c;
-      if (cidx < cols) {
+      if (FULL || cidx < cols) {
 #pragma unroll
         for (int m = 0; m < RR; m++) {
           const uint32_t j = row_of<LO, R>(u, m);
@@ -181,10 +181,10 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
     }
     if constexpr (KIND == PASS_INTT) {
       dft_reg<R>(x, rinv);
-      const uint32_t low = u & ((1u << LO) - 1u);
+      const Tw* tw = own + ((u & ((1u << LO) - 1u)) << R);
       x[0] = reduce(x[0]);  // twiddle w^0 = 1
 #pragma unroll
-      for (int k = 1; k < RR; k++) x[k] = mulmod(x[k], own[((uint32_t)k << LO) | low]);
+      for (int k = 1; k < RR; k++) x[k] = mulmod(x[k], tw[k]);
     } else {
       if constexpr (KIND == PASS_MID) {
         dft_reg<R>(x, rinv);
@@ -204,7 +204,7 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
     }
     if constexpr (LAST) {
       const uint32_t cidx = tile * PL::W + c;
-      if (cidx < cols) {
+      if (FULL || cidx < cols) {
 #pragma unroll
         for (int m = 0; m < RR; m++) {
           const uint32_t j = row_of<LO, R>(u, m);
@@ -219,40 +219,41 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
 }
 
 // Step STEP (0 .. 2D-2) of the encode of one tile; steps are separated by a
-// work-group barrier on the GPU.
-template <class PL, int STEP>
+// work-group barrier on the GPU.  FULL: every column of the tile exists and
+// k = 2^L, so no bounds checks are emitted.
+template <class PL, int STEP, bool FULL>
 ZHD void encode_step(int tid, uint32_t tile, const uint32_t* __restrict__ in,
                      uint32_t* __restrict__ out, uint32_t krows, uint32_t cols, int layout,
                      int32_t* __restrict__ lds, const EncodeTabs& T) {
   constexpr int L1 = PL::LO1, L2 = PL::LO2;
   if constexpr (PL::D == 1) {
-    encode_pass<PL, 0, PL::RD, PASS_MID, true, true, 0, 0>(tid, tile, in, out, krows, cols, layout,
+    encode_pass<PL, 0, PL::RD, PASS_MID, true, true, 0, 0, FULL>(tid, tile, in, out, krows, cols, layout,
                                                            lds, T, nullptr, nullptr);
   } else if constexpr (PL::D == 2) {
     if constexpr (STEP == 0)
-      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0>(tid, tile, in, out, krows, cols,
+      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL>(tid, tile, in, out, krows, cols,
                                                                 layout, lds, T, T.inv1, nullptr);
     else if constexpr (STEP == 1)
-      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L1, PL::R1>(
+      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L1, PL::R1, FULL>(
           tid, tile, in, out, krows, cols, layout, lds, T, nullptr, T.fwd1);
     else
-      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0>(tid, tile, in, out, krows, cols,
+      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL>(tid, tile, in, out, krows, cols,
                                                                layout, lds, T, nullptr, nullptr);
   } else {
     if constexpr (STEP == 0)
-      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0>(tid, tile, in, out, krows, cols,
+      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL>(tid, tile, in, out, krows, cols,
                                                                 layout, lds, T, T.inv1, nullptr);
     else if constexpr (STEP == 1)
-      encode_pass<PL, L2, PL::R2, PASS_INTT, false, false, 0, 0>(tid, tile, in, out, krows, cols,
+      encode_pass<PL, L2, PL::R2, PASS_INTT, false, false, 0, 0, FULL>(tid, tile, in, out, krows, cols,
                                                                  layout, lds, T, T.inv2, nullptr);
     else if constexpr (STEP == 2)
-      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L2, PL::R2>(
+      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L2, PL::R2, FULL>(
           tid, tile, in, out, krows, cols, layout, lds, T, nullptr, T.fwd2);
     else if constexpr (STEP == 3)
-      encode_pass<PL, L2, PL::R2, PASS_NTT, false, false, L1, PL::R1>(
+      encode_pass<PL, L2, PL::R2, PASS_NTT, false, false, L1, PL::R1, FULL>(
           tid, tile, in, out, krows, cols, layout, lds, T, nullptr, T.fwd1);
     else
-      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0>(tid, tile, in, out, krows, cols,
+      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL>(tid, tile, in, out, krows, cols,
                                                                layout, lds, T, nullptr, nullptr);
   }
 }

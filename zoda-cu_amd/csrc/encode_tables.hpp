@@ -18,16 +18,22 @@ struct EncodeTabOffsets {
 
 inline Tw make_tw(uint32_t w) { return Tw{(double)w, (double)w / PD}; }
 
-// Digit of R bits at bit LO of a size-2^(LO+R) sub-transform:
-// tab[k << LO | low] = w_(2^(LO+R))^(+-low*k).
+// Digit of R bits at bit LO of a size-2^(LO+R) sub-transform, value
+// w_(2^(LO+R))^(+-low*k).  The inverse tables are read by the INTT pass of the
+// digit itself, which needs all k for one `low`: laid out [low][k].  The
+// forward tables are read by the pass below, which needs consecutive `low`
+// for one k: laid out [k][low].
 inline void push_digit_table(std::vector<Tw>& v, int lo, int r, bool inverse) {
   uint32_t w = root_of_unity((uint32_t)(lo + r));
   if (inverse) w = inv_mod_host(w);
+  const size_t base = v.size();
+  v.resize(base + ((size_t)1 << (lo + r)));
   for (uint32_t k = 0; k < (1u << r); k++) {
     const uint32_t step = pow_mod_host(w, k);  // w^k
     uint32_t cur = 1;
     for (uint32_t low = 0; low < (1u << lo); low++) {
-      v.push_back(make_tw(cur));
+      const size_t idx = inverse ? ((size_t)low << r) | k : ((size_t)k << lo) | low;
+      v[base + idx] = make_tw(cur);
       cur = mul_mod_host(cur, step);
     }
   }

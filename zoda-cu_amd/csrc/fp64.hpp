@@ -55,9 +55,8 @@ ZHD double reduce(double x) {
 
 // |x| < 2^51 -> canonical value in [0, p).
 ZHD uint32_t canonical(double x) {
-  double r = reduce(x);
-  r = r < 0.0 ? r + PD : r;  // |r| <= p/2 + 1, so one correction suffices
-  return (uint32_t)r;
+  const int32_t r = (int32_t)reduce(x);  // |r| <= p/2 + 1, exact in int32
+  return (uint32_t)r + ((uint32_t)(r >> 31) & 2013265921u);
 }
 
 }  // namespace zoda
