@@ -105,6 +105,43 @@ int zoda_hip_encode_host(const uint32_t* h_in, uint32_t k, uint32_t cols, uint32
  * v = z >> 33; the first v < p is the value.  Asynchronous on `stream`. */
 int zoda_hip_fill_uniform(uint32_t* d_out, size_t count, uint64_t seed, void* stream);
 
+/* ---- batched encode (no counterpart in the reference) --------------------- */
+
+/* One square of a batch: device buffers, row-major in and out (n_kn x cols
+ * out, n_kn = next_pow2(k + n) = 2 * next_pow2(k)); k <= 4096. */
+typedef struct zoda_square {
+  const uint32_t* in;
+  uint32_t* out;
+  uint32_t k;
+  uint32_t cols;
+  uint32_t n;
+  uint32_t reserved; /* 0 */
+} zoda_square;
+
+/* Encodes `count` squares of any mix of sides in one persistent launch that
+ * drains a device work queue of (square, column tile) items, largest side
+ * first.  Synchronous on `stream`.  Results equal zoda_hip_encode_device of
+ * each square alone. */
+int zoda_hip_encode_batch(const zoda_square* squares, uint32_t count, void* stream);
+
+/* A batch captured once into a hipGraph ([queue reset, persistent kernel])
+ * and replayed by zoda_hip_batch_plan_launch (asynchronous on `stream`).  The
+ * squares' buffers must outlive the plan; their contents may change between
+ * replays. */
+typedef struct zoda_batch_plan zoda_batch_plan;
+int zoda_hip_batch_plan_create(const zoda_square* squares, uint32_t count, zoda_batch_plan** plan);
+int zoda_hip_batch_plan_launch(zoda_batch_plan* plan, void* stream);
+int zoda_hip_batch_plan_destroy(zoda_batch_plan* plan);
+
+/* Autotuner of the batched kernel: for each side (a power of two <= 4096),
+ * times every plan variant (tile width, radix, stages per register pass) on a
+ * synthetic side x 128*side square and keeps the fastest for this device.
+ * times_us, if not NULL, receives count * 2 timings.  get/set read and
+ * override the table entry of log2(side). */
+int zoda_hip_autotune(const uint32_t* sides, uint32_t count, int iters, float* times_us);
+int zoda_hip_get_variant(uint32_t log_side, int* variant);
+int zoda_hip_set_variant(uint32_t log_side, int variant);
+
 #ifdef __cplusplus
 }
 #endif

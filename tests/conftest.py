@@ -10,6 +10,12 @@ for sub in ("zoda-cu_amd", "oracle"):
         sys.path.insert(0, path)
 
 
+try:
+    import torch  # noqa: F401  (load torch's HIP runtime before libzoda_hip; see zoda_hip)
+except ImportError:
+    pass
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
     config.addinivalue_line("markers", "slow: long-running CPU test")

@@ -89,11 +89,14 @@ def sim_lib():
         lib.zoda_sim_encode.restype = ctypes.c_int
         lib.zoda_sim_encode.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                         u32p, ctypes.c_int]
+        lib.zoda_sim_encode_variant.restype = ctypes.c_int
+        lib.zoda_sim_encode_variant.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                u32p, ctypes.c_int, ctypes.c_int]
         _sim = lib
     return _sim
 
 
-def sim_encode(data: np.ndarray, n: int | None = None, layout: int = 0) -> np.ndarray:
+def sim_encode(data: np.ndarray, n: int | None = None, layout: int = 0, variant: int = -1) -> np.ndarray:
     k, cols = data.shape
     if n is None:
         n = k
@@ -102,7 +105,7 @@ def sim_encode(data: np.ndarray, n: int | None = None, layout: int = 0) -> np.nd
         nkn <<= 1
     out = np.zeros((nkn, cols) if layout == 0 else (cols, nkn), dtype=np.uint32)
     d = np.ascontiguousarray(data, dtype=np.uint32)
-    rc = sim_lib().zoda_sim_encode(d.ctypes.data, k, cols, n, out.ctypes.data, layout)
+    rc = sim_lib().zoda_sim_encode_variant(d.ctypes.data, k, cols, n, out.ctypes.data, layout, variant)
     if rc != 0:
         raise ValueError(f"shape not covered by the fused kernel (k={k}, n={n})")
     return out

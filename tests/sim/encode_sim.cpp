@@ -24,6 +24,39 @@ static void run_steps(uint32_t tile, const uint32_t* in, uint32_t* out, uint32_t
     run_steps<PL, STEP + 1, FULL>(tile, in, out, krows, cols, layout, lds, T);
 }
 
+template <class PL>
+static int sim_plan(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out, int layout) {
+  EncodeTabOffsets off;
+  std::vector<Tw> tabs = build_encode_tables(PL::L, PL::R1, PL::R2, PL::RD, &off);
+  EncodeTabs T{tabs.data() + off.inv1, tabs.data() + off.inv2, tabs.data() + off.fwd1,
+               tabs.data() + off.fwd2, tabs.data() + off.twist, tabs.data() + off.roots};
+  std::vector<int32_t> lds(PL::LDS_WORDS ? PL::LDS_WORDS : 1);
+  const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
+  const bool full = cols % PL::W == 0 && k == (uint32_t)PL::S;
+  for (uint32_t tile = 0; tile < ntiles; tile++) {
+    if (full)
+      run_steps<PL, 0, true>(tile, in, out, k, cols, layout, lds.data(), T);
+    else
+      run_steps<PL, 0, false>(tile, in, out, k, cols, layout, lds.data(), T);
+  }
+  return 0;
+}
+
+// variant < 0: the single-square plan of the side; 0/1: the batched plans.
+extern "C" int zoda_sim_encode_variant(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t n,
+                                       uint32_t* out, int layout, int variant) {
+  uint32_t nk = 1, L = 0;
+  while (nk < k) { nk <<= 1; L++; }
+  if (k == 0 || cols == 0 || k + n <= nk || k + n > 2 * nk) return -1;
+  if (variant < 0)
+    return with_encode_plan((int)L, [&](auto plan) -> int {
+      return sim_plan<decltype(plan)>(in, k, cols, out, layout);
+    });
+  return with_batch_plan((int)L, variant, [&](auto plan) -> int {
+    return sim_plan<decltype(plan)>(in, k, cols, out, layout);
+  });
+}
+
 extern "C" int zoda_sim_encode(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t n,
                                uint32_t* out, int layout) {
   uint32_t nk = 1, L = 0;

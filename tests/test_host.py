@@ -113,3 +113,13 @@ def test_kernel_emulation_full_tiles(k, cols):
     rng = np.random.default_rng(k + cols)
     data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)
     assert np.array_equal(sim_encode(data), O.encode(data, threads=4))
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("k", [1, 3, 8, 16, 17, 32, 64, 100, 128, 256, 512, 1024, 2048, 4096])
+def test_batched_plans_emulation(k, variant):
+    # the plans of the persistent batched kernel (both autotuner variants)
+    rng = np.random.default_rng(10 * k + variant)
+    cols = 70 if k <= 1024 else 12
+    data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)
+    assert np.array_equal(sim_encode(data, variant=variant), O.encode(data, threads=4))

@@ -1,0 +1,339 @@
+// Batched encode of many squares of uneven sides in one persistent launch.
+//
+// Work items are (square, column tile) pairs, ordered largest side first.  A
+// grid sized to fill every CU drains them through one atomic counter: each
+// work-group takes an item, runs all passes of that tile with the plan of the
+// square's side (every pass group of a tile is fused, the tile being
+// LDS-resident), and takes the next.  The counter is reset by a one-thread
+// kernel, so [reset, persistent kernel] can be captured once into a hipGraph
+// and replayed (zoda_hip_batch_plan_*).  The plan variant of each side comes
+// from the per-device autotune table (zoda_hip_autotune).
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+#include "encode_plans.hpp"
+#include "encode_tables.hpp"
+
+namespace zoda {
+
+struct SquareDesc {
+  const uint32_t* in;
+  uint32_t* out;
+  uint32_t k, cols, L, variant, full, tiles;
+};
+
+struct BatchItem {
+  uint32_t square, tile;
+};
+
+constexpr int MAX_LOG = 16;
+
+// Not inlined: each plan is register-allocated on its own, so the kernel's
+// VGPR budget is the largest plan's rather than a merge of all of them.
+template <class PL, bool FULL>
+__device__ __attribute__((noinline)) void batch_tile_steps(int tid, uint32_t tile, const SquareDesc& d,
+                                                 int32_t* lds, const EncodeTabs& T) {
+  encode_step<PL, 0, FULL>(tid, tile, d.in, d.out, d.k, d.cols, 0, lds, T);
+  if constexpr (PL::STEPS > 1) {
+    __syncthreads();
+    encode_step<PL, 1, FULL>(tid, tile, d.in, d.out, d.k, d.cols, 0, lds, T);
+    __syncthreads();
+    encode_step<PL, 2, FULL>(tid, tile, d.in, d.out, d.k, d.cols, 0, lds, T);
+  }
+  if constexpr (PL::STEPS > 3) {
+    __syncthreads();
+    encode_step<PL, 3, FULL>(tid, tile, d.in, d.out, d.k, d.cols, 0, lds, T);
+    __syncthreads();
+    encode_step<PL, 4, FULL>(tid, tile, d.in, d.out, d.k, d.cols, 0, lds, T);
+  }
+}
+
+__global__ void batch_reset(uint32_t* counter) { *counter = 0u; }
+
+__global__ __launch_bounds__(BATCH_NT) void batch_kernel(const SquareDesc* __restrict__ squares,
+                                                         const BatchItem* __restrict__ items,
+                                                         uint32_t nitems, uint32_t* counter,
+                                                         const EncodeTabs* __restrict__ tabs) {
+  extern __shared__ int32_t lds[];
+  __shared__ uint32_t s_item;
+  const int tid = (int)threadIdx.x;
+  for (;;) {
+    if (tid == 0) s_item = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t it = s_item;
+    __syncthreads();
+    if (it >= nitems) break;  // uniform: every wave sees the same item
+    const BatchItem bi = items[it];
+    const SquareDesc d = squares[bi.square];
+    const EncodeTabs T = tabs[d.L * BATCH_VARIANTS + d.variant];
+    with_batch_plan((int)d.L, (int)d.variant, [&](auto plan) -> int {
+      using PL = decltype(plan);
+      if (d.full)
+        batch_tile_steps<PL, true>(tid, bi.tile, d, lds, T);
+      else
+        batch_tile_steps<PL, false>(tid, bi.tile, d, lds, T);
+      return 0;
+    });
+  }
+}
+
+}  // namespace zoda
+
+using namespace zoda;
+
+struct zoda_batch_plan {
+  int device = 0;
+  SquareDesc* d_squares = nullptr;
+  BatchItem* d_items = nullptr;
+  EncodeTabs* d_tabs = nullptr;
+  uint32_t* d_counter = nullptr;
+  uint32_t nitems = 0, grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static void free_plan(zoda_batch_plan* p) {
+  if (!p) return;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  (void)hipFree(p->d_squares);
+  (void)hipFree(p->d_items);
+  (void)hipFree(p->d_tabs);
+  (void)hipFree(p->d_counter);
+  delete p;
+}
+
+static int check_square(const zoda_square& s, uint32_t i) {
+  if (!s.in || !s.out) return fail(ZODA_EINVAL, "square %u: NULL buffer", i);
+  if (s.k == 0 || s.cols == 0) return fail(ZODA_EINVAL, "square %u: k and cols must be positive", i);
+  const uint32_t L = log2_ceil(s.k);
+  if ((int)L > BATCH_MAX_LOG)
+    return fail(ZODA_EINVAL, "square %u: k = %u exceeds the batched maximum 2^%d", i, s.k, BATCH_MAX_LOG);
+  if (log2_ceil((uint64_t)s.k + s.n) != L + 1)
+    return fail(ZODA_EINVAL, "square %u: next_pow2(k + n) must be 2 * next_pow2(k)", i);
+  return 0;
+}
+
+static int launch_batch_direct(zoda_batch_plan* p, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_reset, dim3(1), dim3(1), 0, stream, p->d_counter);
+  ZODA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(batch_kernel, dim3(p->grid), dim3(BATCH_NT), BATCH_LDS_WORDS * 4, stream,
+                     p->d_squares, p->d_items, p->nitems, p->d_counter, p->d_tabs);
+  ZODA_HIP(hipGetLastError());
+  return 0;
+}
+
+// Builds descriptors, the largest-first item list and the per-(side, variant)
+// table pointers on the device; with `graph`, also captures [reset, kernel].
+static int build_plan(const zoda_square* squares, uint32_t count, bool graph, zoda_batch_plan** out) {
+  clear_error();
+  if (!out) return fail(ZODA_EINVAL, "plan pointer is NULL");
+  *out = nullptr;
+  if (!squares || count == 0) return fail(ZODA_EINVAL, "no squares");
+  for (uint32_t i = 0; i < count; i++) ZODA_TRY(check_square(squares[i], i));
+  DeviceState* st = nullptr;
+  ZODA_TRY(current_state(&st));
+  std::lock_guard<std::recursive_mutex> lock(st->mu);
+
+  std::vector<SquareDesc> descs(count);
+  std::vector<EncodeTabs> tabs((BATCH_MAX_LOG + 1) * BATCH_VARIANTS, EncodeTabs{});
+  std::vector<bool> have(tabs.size(), false);
+  uint64_t nitems = 0;
+  for (uint32_t i = 0; i < count; i++) {
+    const zoda_square& s = squares[i];
+    SquareDesc& d = descs[i];
+    d.in = s.in;
+    d.out = s.out;
+    d.k = s.k;
+    d.cols = s.cols;
+    d.L = log2_ceil(s.k);
+    d.variant = (uint32_t)st->batch_variant[d.L];
+    int rc = with_batch_plan((int)d.L, (int)d.variant, [&](auto plan) -> int {
+      using PL = decltype(plan);
+      d.tiles = (s.cols + PL::W - 1) / PL::W;
+      d.full = (s.cols % PL::W == 0 && s.k == (uint32_t)PL::S) ? 1u : 0u;
+      const size_t slot = (size_t)d.L * BATCH_VARIANTS + d.variant;
+      if (!have[slot]) {
+        const EncodeTableSet* set = nullptr;
+        ZODA_TRY(get_encode_tables(st, PL::L, PL::R1, PL::R2, PL::RD, &set));
+        tabs[slot] = EncodeTabs{set->base + set->inv1, set->base + set->inv2, set->base + set->fwd1,
+                                set->base + set->fwd2, set->base + set->twist, set->base + set->roots};
+        have[slot] = true;
+      }
+      return 0;
+    });
+    if (rc) return rc;
+    nitems += d.tiles;
+  }
+  if (nitems >= (1ull << 32)) return fail(ZODA_EINVAL, "batch of %llu tiles is too large", (unsigned long long)nitems);
+  // Largest sides first so the long tiles start early and the short ones fill the tail.
+  std::vector<uint32_t> order(count);
+  for (uint32_t i = 0; i < count; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return descs[a].L > descs[b].L; });
+  std::vector<BatchItem> items;
+  items.reserve(nitems);
+  for (uint32_t i : order)
+    for (uint32_t t = 0; t < descs[i].tiles; t++) items.push_back(BatchItem{i, t});
+
+  zoda_batch_plan* p = new zoda_batch_plan();
+  p->device = st->device;
+  p->nitems = (uint32_t)items.size();
+  auto bail = [&](int rc) {
+    free_plan(p);
+    return rc;
+  };
+  if (hipMalloc((void**)&p->d_squares, descs.size() * sizeof(SquareDesc)) != hipSuccess ||
+      hipMalloc((void**)&p->d_items, items.size() * sizeof(BatchItem)) != hipSuccess ||
+      hipMalloc((void**)&p->d_tabs, tabs.size() * sizeof(EncodeTabs)) != hipSuccess ||
+      hipMalloc((void**)&p->d_counter, sizeof(uint32_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    return bail(fail(ZODA_ENOMEM, "batch plan allocation failed"));
+  }
+  hipError_t e = hipMemcpy(p->d_squares, descs.data(), descs.size() * sizeof(SquareDesc), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->d_tabs, tabs.data(), tabs.size() * sizeof(EncodeTabs), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return bail(fail((int)e, "batch plan upload failed: %s", hipGetErrorString(e)));
+
+  int per_cu = 0, cus = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel, BATCH_NT, BATCH_LDS_WORDS * 4);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, st->device);
+  if (e != hipSuccess) return bail(fail((int)e, "occupancy query failed: %s", hipGetErrorString(e)));
+  p->grid = (uint32_t)std::max(1, per_cu * cus);
+  p->grid = std::min<uint32_t>(p->grid, p->nitems);
+
+  if (graph) {
+    hipStream_t cap;
+    e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(fail((int)e, "stream creation failed: %s", hipGetErrorString(e)));
+    e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    int rc = e == hipSuccess ? launch_batch_direct(p, cap) : fail((int)e, "begin capture failed");
+    hipError_t e2 = hipStreamEndCapture(cap, &p->graph);
+    (void)hipStreamDestroy(cap);
+    if (rc) return bail(rc);
+    if (e2 != hipSuccess) return bail(fail((int)e2, "end capture failed: %s", hipGetErrorString(e2)));
+    e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return bail(fail((int)e, "graph instantiation failed: %s", hipGetErrorString(e)));
+  }
+  *out = p;
+  return 0;
+}
+
+extern "C" int zoda_hip_batch_plan_create(const zoda_square* squares, uint32_t count,
+                                          zoda_batch_plan** plan) {
+  return build_plan(squares, count, true, plan);
+}
+
+extern "C" int zoda_hip_batch_plan_launch(zoda_batch_plan* plan, void* stream) {
+  clear_error();
+  if (!plan || !plan->exec) return fail(ZODA_EINVAL, "invalid batch plan");
+  ZODA_HIP(hipGraphLaunch(plan->exec, (hipStream_t)stream));
+  return 0;
+}
+
+extern "C" int zoda_hip_batch_plan_destroy(zoda_batch_plan* plan) {
+  clear_error();
+  if (!plan) return 0;
+  ZODA_HIP(hipDeviceSynchronize());  // a replay may still be reading the plan
+  free_plan(plan);
+  return 0;
+}
+
+extern "C" int zoda_hip_encode_batch(const zoda_square* squares, uint32_t count, void* stream) {
+  zoda_batch_plan* p = nullptr;
+  ZODA_TRY(build_plan(squares, count, false, &p));
+  int rc = launch_batch_direct(p, (hipStream_t)stream);
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  free_plan(p);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail((int)e, "batch encode failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
+extern "C" int zoda_hip_get_variant(uint32_t log_side, int* variant) {
+  clear_error();
+  if (!variant || (int)log_side > BATCH_MAX_LOG) return fail(ZODA_EINVAL, "bad arguments");
+  DeviceState* st = nullptr;
+  ZODA_TRY(current_state(&st));
+  *variant = st->batch_variant[log_side];
+  return 0;
+}
+
+extern "C" int zoda_hip_set_variant(uint32_t log_side, int variant) {
+  clear_error();
+  if ((int)log_side > BATCH_MAX_LOG || variant < 0 || variant >= BATCH_VARIANTS)
+    return fail(ZODA_EINVAL, "bad arguments");
+  DeviceState* st = nullptr;
+  ZODA_TRY(current_state(&st));
+  std::lock_guard<std::recursive_mutex> lock(st->mu);
+  st->batch_variant[log_side] = variant;
+  return 0;
+}
+
+// For each listed side, times every batched-plan variant on one synthetic
+// square of that side (side rows x 128*side columns, device-resident, the best
+// of `iters` launches of a one-square batch) and keeps the fastest in the
+// device's table.  times_us (optional) receives count * BATCH_VARIANTS values.
+extern "C" int zoda_hip_autotune(const uint32_t* sides, uint32_t count, int iters, float* times_us) {
+  clear_error();
+  if (!sides || count == 0 || iters < 1) return fail(ZODA_EINVAL, "bad arguments");
+  DeviceState* st = nullptr;
+  ZODA_TRY(current_state(&st));
+  hipStream_t s;
+  ZODA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  ZODA_HIP(hipEventCreate(&e0));
+  ZODA_HIP(hipEventCreate(&e1));
+  int rc = 0;
+  for (uint32_t i = 0; i < count && !rc; i++) {
+    const uint32_t side = sides[i];
+    if (!is_pow2(side) || (int)log2_ceil(side) > BATCH_MAX_LOG) {
+      rc = fail(ZODA_EINVAL, "side %u must be a power of two <= 2^%d", side, BATCH_MAX_LOG);
+      break;
+    }
+    const uint32_t L = log2_ceil(side);
+    const uint64_t cols = 128ull * side;
+    uint32_t *din = nullptr, *dout = nullptr;
+    if (hipMalloc((void**)&din, side * cols * 4) != hipSuccess ||
+        hipMalloc((void**)&dout, 2 * side * cols * 4) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(din);
+      rc = fail(ZODA_ENOMEM, "autotune buffers for side %u", side);
+      break;
+    }
+    rc = launch_fill_uniform(din, side * cols, side, s);
+    float best = 1e30f;
+    int best_v = 0;
+    for (int v = 0; v < BATCH_VARIANTS && !rc; v++) {
+      st->batch_variant[L] = v;
+      zoda_square sq{din, dout, side, (uint32_t)cols, side, 0};
+      zoda_batch_plan* p = nullptr;
+      rc = build_plan(&sq, 1, false, &p);
+      if (rc) break;
+      float vbest = 1e30f;
+      for (int it = 0; it <= iters && !rc; it++) {
+        hipEventRecord(e0, s);
+        rc = launch_batch_direct(p, s);
+        hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = fail(ZODA_ESTATE, "autotune launch failed");
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (it > 0) vbest = std::min(vbest, ms * 1000.0f);  // first launch warms up
+      }
+      free_plan(p);
+      if (times_us) times_us[i * BATCH_VARIANTS + v] = vbest;
+      if (vbest < best) {
+        best = vbest;
+        best_v = v;
+      }
+    }
+    st->batch_variant[L] = best_v;
+    (void)hipFree(din);
+    (void)hipFree(dout);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  return rc;
+}

@@ -59,7 +59,8 @@ struct DeviceState {
   int device = 0;
   std::recursive_mutex mu;
   std::map<uint32_t, MontTables> mont;           // standard roots, keyed by log n
-  std::map<uint32_t, EncodeTableSet> encode;     // keyed by log n_k
+  std::map<uint32_t, EncodeTableSet> encode;     // keyed by encode_table_key()
+  int batch_variant[16] = {0};                   // autotuned batched-plan variant per log side
   // Drop-in entry points: tables built from the caller's roots.
   std::map<uint32_t, std::vector<uint32_t>> user_roots;
   std::map<uint32_t, MontTables> user_mont;
@@ -82,6 +83,12 @@ int get_user_mont_tables(DeviceState* st, const uint32_t* roots, uint32_t log_n,
 // Batched poly-major transform on device memory (ntt_poly.hip).
 int launch_ntt_poly(uint32_t* d_values, uint32_t log_n, uint32_t polys, uint32_t stride,
                     int inverse, const MontTables& tabs, hipStream_t stream);
+
+inline uint32_t encode_table_key(int L, int r1, int r2, int rd) {
+  return (uint32_t)L | ((uint32_t)r1 << 8) | ((uint32_t)r2 << 16) | ((uint32_t)rd << 24);
+}
+// Constant tables of the fused encode for side 2^L and digit split (r1, r2, rd).
+int get_encode_tables(DeviceState* st, int L, int r1, int r2, int rd, const EncodeTableSet** out);
 
 // Fused column encode on device memory (encode.hip).  in_pitch/out_pitch are
 // the row pitches (elements) of the row-major input and row-major output; for

@@ -40,10 +40,10 @@ __global__ __launch_bounds__(PL::NT) void encode_kernel(const uint32_t* __restri
   }
 }
 
-static int get_encode_tables(DeviceState* st, int L, int r1, int r2, int rd,
-                             const EncodeTableSet** out) {
+int get_encode_tables(DeviceState* st, int L, int r1, int r2, int rd, const EncodeTableSet** out) {
   std::lock_guard<std::recursive_mutex> lock(st->mu);
-  auto it = st->encode.find((uint32_t)L);
+  const uint32_t key = encode_table_key(L, r1, r2, rd);
+  auto it = st->encode.find(key);
   if (it == st->encode.end()) {
     EncodeTabOffsets off;
     std::vector<Tw> host = build_encode_tables(L, r1, r2, rd, &off);
@@ -60,7 +60,7 @@ static int get_encode_tables(DeviceState* st, int L, int r1, int r2, int rd,
     set.fwd2 = off.fwd2;
     set.twist = off.twist;
     set.roots = off.roots;
-    it = st->encode.emplace((uint32_t)L, set).first;
+    it = st->encode.emplace(key, set).first;
   }
   *out = &it->second;
   return 0;

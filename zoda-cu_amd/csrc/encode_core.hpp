@@ -138,37 +138,42 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
   const Tw* rinv = T.roots + 32;
   const Tw* rfwd = T.roots;
 
-  uint32_t raw[FIRST ? G : 1][FIRST ? RR : 1];
-  if constexpr (FIRST) {
+  // The first pass loads up to 32 values per thread before computing, in
+  // chunks of PB groups, to keep loads in flight without a large register set.
+  constexpr int PB = FIRST ? (G * RR <= 32 ? G : (RR >= 32 ? 1 : 32 / RR)) : G;
+  static_assert(G % PB == 0, "preload chunks must divide the groups");
 #pragma unroll
-    for (int gi = 0; gi < G; gi++) {
-      const uint32_t g = (uint32_t)(gi * PL::NT + tid);
-      const uint32_t c = g % PL::W, u = g / PL::W;
-      const uint32_t cidx = tile * PL::W + c;
+  for (int gc = 0; gc < G; gc += PB) {
+    uint32_t raw[FIRST ? PB : 1][FIRST ? RR : 1];
+    if constexpr (FIRST) {
 #pragma unroll
-      for (int m = 0; m < RR; m++) {
-        const uint32_t j = row_of<LO, R>(u, m);
-        raw[gi][m] = (FULL || (cidx < cols && j < krows)) ? in[(size_t)j * cols + cidx] : 0u;
-      }
-    }
-#pragma unroll
-    for (int gi = 0; gi < G; gi++) {
-      const uint32_t g = (uint32_t)(gi * PL::NT + tid);
-      const uint32_t c = g % PL::W, u = g / PL::W;
-      const uint32_t cidx = tile * PL::W + c;
-      if (FULL || cidx < cols) {
+      for (int gi = 0; gi < PB; gi++) {
+        const uint32_t g = (uint32_t)((gc + gi) * PL::NT + tid);
+        const uint32_t c = g % PL::W, u = g / PL::W;
+        const uint32_t cidx = tile * PL::W + c;
 #pragma unroll
         for (int m = 0; m < RR; m++) {
           const uint32_t j = row_of<LO, R>(u, m);
-          out[out_index(2 * j, cidx, cols, OUT_ROWS, layout)] = raw[gi][m];
+          raw[gi][m] = (FULL || (cidx < cols && j < krows)) ? in[(size_t)j * cols + cidx] : 0u;
+        }
+      }
+#pragma unroll
+      for (int gi = 0; gi < PB; gi++) {
+        const uint32_t g = (uint32_t)((gc + gi) * PL::NT + tid);
+        const uint32_t c = g % PL::W, u = g / PL::W;
+        const uint32_t cidx = tile * PL::W + c;
+        if (FULL || cidx < cols) {
+#pragma unroll
+          for (int m = 0; m < RR; m++) {
+            const uint32_t j = row_of<LO, R>(u, m);
+            out[out_index(2 * j, cidx, cols, OUT_ROWS, layout)] = raw[gi][m];
+          }
         }
       }
     }
-  }
-
 #pragma unroll
-  for (int gi = 0; gi < G; gi++) {
-    const uint32_t g = (uint32_t)(gi * PL::NT + tid);
+  for (int gi = 0; gi < PB; gi++) {
+    const uint32_t g = (uint32_t)((gc + gi) * PL::NT + tid);
     const uint32_t c = g % PL::W, u = g / PL::W;
     double x[RR];
 #pragma unroll
@@ -215,6 +220,7 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
 #pragma unroll
       for (int m = 0; m < RR; m++) lds[row_of<LO, R>(u, m) * PL::W + c] = (int32_t)x[m];
     }
+  }
   }
 }
 

@@ -46,3 +46,68 @@ int with_encode_plan(int L, F&& f) {
 }
 
 }  // namespace zoda
+
+namespace zoda {
+
+// Plans of the batched persistent kernel: one block size (256) for every
+// side, LDS <= 64 KiB so two work-groups share a CU, and two variants per
+// side (tile width W, digit split = radix and stages per register pass) for
+// the autotuner to choose from.
+constexpr int BATCH_NT = 256;
+constexpr int BATCH_MIN_LOG = 0;
+constexpr int BATCH_MAX_LOG = 12;
+constexpr int BATCH_VARIANTS = 2;
+constexpr int BATCH_LDS_WORDS = 16384;
+
+//                          L   W    NT        R1 R2 RD
+using B0a = EncodePlan<0, 256, BATCH_NT, 0, 0, 0>;
+using B1a = EncodePlan<1, 256, BATCH_NT, 0, 0, 1>;
+using B2a = EncodePlan<2, 256, BATCH_NT, 0, 0, 2>;
+using B3a = EncodePlan<3, 256, BATCH_NT, 0, 0, 3>;
+using B4a = EncodePlan<4, 256, BATCH_NT, 0, 0, 4>;
+using B4b = EncodePlan<4, 64, BATCH_NT, 2, 0, 2>;
+using B5a = EncodePlan<5, 256, BATCH_NT, 0, 0, 5>;
+using B5b = EncodePlan<5, 64, BATCH_NT, 3, 0, 2>;
+using B6a = EncodePlan<6, 64, BATCH_NT, 3, 0, 3>;
+using B6b = EncodePlan<6, 32, BATCH_NT, 3, 0, 3>;
+using B7a = EncodePlan<7, 64, BATCH_NT, 4, 0, 3>;
+using B7b = EncodePlan<7, 32, BATCH_NT, 4, 0, 3>;
+using B8a = EncodePlan<8, 64, BATCH_NT, 4, 0, 4>;
+using B8b = EncodePlan<8, 32, BATCH_NT, 3, 2, 3>;
+using B9a = EncodePlan<9, 32, BATCH_NT, 3, 3, 3>;
+using B9b = EncodePlan<9, 32, BATCH_NT, 5, 0, 4>;
+using B10a = EncodePlan<10, 16, BATCH_NT, 4, 3, 3>;
+using B10b = EncodePlan<10, 16, BATCH_NT, 5, 0, 5>;
+using B11a = EncodePlan<11, 8, BATCH_NT, 4, 4, 3>;
+using B12a = EncodePlan<12, 4, BATCH_NT, 4, 4, 4>;
+
+// Calls f(Plan{}) for batched plan (L, variant); variants a side lacks fall
+// back to variant 0.  Returns -1 for L out of range.
+template <class F>
+ZHD int with_batch_plan(int L, int variant, F&& f) {
+  switch (L * BATCH_VARIANTS + (variant & 1)) {
+    case 0: case 1: return f(B0a{});
+    case 2: case 3: return f(B1a{});
+    case 4: case 5: return f(B2a{});
+    case 6: case 7: return f(B3a{});
+    case 8: return f(B4a{});
+    case 9: return f(B4b{});
+    case 10: return f(B5a{});
+    case 11: return f(B5b{});
+    case 12: return f(B6a{});
+    case 13: return f(B6b{});
+    case 14: return f(B7a{});
+    case 15: return f(B7b{});
+    case 16: return f(B8a{});
+    case 17: return f(B8b{});
+    case 18: return f(B9a{});
+    case 19: return f(B9b{});
+    case 20: return f(B10a{});
+    case 21: return f(B10b{});
+    case 22: case 23: return f(B11a{});
+    case 24: case 25: return f(B12a{});
+    default: return -1;
+  }
+}
+
+}  // namespace zoda

@@ -21,6 +21,15 @@ import os
 
 import numpy as np
 
+try:
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7).
+    # Loading it first makes libzoda_hip bind to that copy, so torch and this
+    # library share one runtime (two copies in one process cannot both open
+    # the device).
+    import torch  # noqa: F401
+except ImportError:  # the C ABI works without torch
+    pass
+
 P = 2013265921
 LAYOUT_ROW = 0
 LAYOUT_COL = 1
@@ -60,6 +69,13 @@ def _load():
         "zoda_hip_encode_host": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                                 _vp, ctypes.c_int]),
         "zoda_hip_fill_uniform": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_uint64, _vp]),
+        "zoda_hip_encode_batch": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp]),
+        "zoda_hip_batch_plan_create": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(_vp)]),
+        "zoda_hip_batch_plan_launch": (ctypes.c_int, [_vp, _vp]),
+        "zoda_hip_batch_plan_destroy": (ctypes.c_int, [_vp]),
+        "zoda_hip_autotune": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_int, _vp]),
+        "zoda_hip_get_variant": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(ctypes.c_int)]),
+        "zoda_hip_set_variant": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -267,3 +283,73 @@ def uniform_numpy(count: int, seed: int) -> np.ndarray:
             if todo.size == 0:
                 break
     return out
+
+
+# ---- batched encode ------------------------------------------------------------------
+
+class Square(ctypes.Structure):
+    """zoda_square: device buffers of one square of a batch."""
+    _fields_ = [("inp", ctypes.c_void_p), ("out", ctypes.c_void_p), ("k", ctypes.c_uint32),
+                ("cols", ctypes.c_uint32), ("n", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+def _squares(specs):
+    arr = (Square * len(specs))()
+    for i, (d_in, d_out, k, cols, n) in enumerate(specs):
+        arr[i] = Square(d_in, d_out, k, cols, n, 0)
+    return arr
+
+
+def encode_batch(specs, stream: int = 0) -> None:
+    """specs: list of (d_in, d_out, k, cols, n) device pointers and shapes."""
+    arr = _squares(specs)
+    _check(lib.zoda_hip_encode_batch(ctypes.addressof(arr), len(specs), stream or None),
+           "zoda_hip_encode_batch")
+
+
+class BatchPlan:
+    """A batch captured into a hipGraph once and replayed with launch()."""
+
+    def __init__(self, specs):
+        self._arr = _squares(specs)
+        self._plan = ctypes.c_void_p()
+        _check(lib.zoda_hip_batch_plan_create(ctypes.addressof(self._arr), len(specs),
+                                              ctypes.byref(self._plan)), "zoda_hip_batch_plan_create")
+
+    def launch(self, stream: int = 0) -> None:
+        _check(lib.zoda_hip_batch_plan_launch(self._plan, stream or None), "zoda_hip_batch_plan_launch")
+
+    def destroy(self) -> None:
+        if self._plan:
+            _check(lib.zoda_hip_batch_plan_destroy(self._plan), "zoda_hip_batch_plan_destroy")
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def autotune(sides, iters: int = 3) -> dict:
+    """Times every batched-plan variant per side and keeps the fastest;
+    returns {side: {"variant": v, "times_us": [...]}}."""
+    sides = list(sides)
+    arr = (ctypes.c_uint32 * len(sides))(*sides)
+    times = (ctypes.c_float * (2 * len(sides)))()
+    _check(lib.zoda_hip_autotune(ctypes.addressof(arr), len(sides), iters, ctypes.addressof(times)),
+           "zoda_hip_autotune")
+    out = {}
+    for i, side in enumerate(sides):
+        out[side] = {"variant": get_variant(side), "times_us": [times[2 * i], times[2 * i + 1]]}
+    return out
+
+
+def get_variant(side: int) -> int:
+    v = ctypes.c_int(0)
+    _check(lib.zoda_hip_get_variant(side.bit_length() - 1, ctypes.byref(v)), "zoda_hip_get_variant")
+    return v.value
+
+
+def set_variant(side: int, variant: int) -> None:
+    _check(lib.zoda_hip_set_variant(side.bit_length() - 1, variant), "zoda_hip_set_variant")
