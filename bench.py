@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-e2e", action="store_true", help="skip the pinned-host end-to-end timing")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-cols", type=int, default=16384, help="columns in the CPU sample")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 96-square batched comparison")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-trace comparison")
+    ap.add_argument("--served-requests", type=int, default=1000)
     return ap.parse_args()
 
 
@@ -80,6 +83,116 @@ def cpu_baseline(side: int, ncols: int, seed: int):
         "single_core_value": round(gbps1, 4),
         "single_core_sample": f"{one} columns on 1 thread ({dt1:.2f} s)",
         "cpu_model": _cpu_model(),
+    }
+
+
+def batched_vs_plain(zoda_hip, torch, seed: int, reps: int = 7):
+    """96 squares of sides drawn over {32..512} (30/30/20/15/5 %), side x 128 side
+    each: one persistent launch replayed from a hipGraph against a plain loop of
+    one encode launch per square on one stream.  Both timed with HIP events."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([32, 64, 128, 256, 512], size=96, p=[0.30, 0.30, 0.20, 0.15, 0.05]).tolist()
+    tuned = zoda_hip.autotune(sorted(set(sides)), iters=3)
+    ins, outs, specs = [], [], []
+    for i, sd in enumerate(sides):
+        t_in = torch.empty((sd, 128 * sd), dtype=torch.int32, device="cuda")
+        zoda_hip.fill_uniform(t_in, seed=seed + i)
+        t_out = torch.empty((2 * sd, 128 * sd), dtype=torch.int32, device="cuda")
+        ins.append(t_in)
+        outs.append(t_out)
+        specs.append((t_in.data_ptr(), t_out.data_ptr(), sd, 128 * sd, sd))
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+
+    def plain():
+        for (di, do, k, c, n) in specs:
+            zoda_hip.encode_device(di, k, c, n, do, 0, sptr)
+
+    plan = zoda_hip.BatchPlan(specs)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms.append(a.elapsed_time(b))
+        ms.sort()
+        return ms[len(ms) // 2]
+
+    t_plain = timed(plain)
+    t_batch = timed(lambda: plan.launch(sptr))
+    plan.destroy()
+    # the two forms agree on the last run
+    ok = True
+    for t_out, (di, do, k, c, n) in zip(outs, specs):
+        ref = torch.empty_like(t_out)
+        zoda_hip.encode_device(di, k, c, n, ref.data_ptr(), 0, sptr)
+        torch.cuda.synchronize()
+        ok = ok and bool(torch.equal(ref, t_out))
+    payload = sum(sd * sd * SHARE_BYTES for sd in sides)
+    return {
+        "workload": "96 squares, sides {32,64,128,256,512} at 30/30/20/15/5 %, side x 128*side u32, device-resident",
+        "payload_bytes": payload,
+        "plain_loop_ms": round(t_plain, 4),
+        "batched_graph_ms": round(t_batch, 4),
+        "plain_loop_gbps": round(payload * 8 / 1e9 / (t_plain / 1e3), 2),
+        "batched_gbps": round(payload * 8 / 1e9 / (t_batch / 1e3), 2),
+        "speedup": round(t_plain / t_batch, 3),
+        "autotuned_variant": {str(k): v["variant"] for k, v in tuned.items()},
+        "bit_exact_vs_plain": ok,
+    }
+
+
+def served_vs_plain(zoda_hip, torch, seed: int, count: int, rates=(2000, 8000)):
+    """Poisson arrivals of squares with log-uniform sides {64..1024} (512 B
+    shares, device-resident payloads drawn from a pool of 12 buffers per side),
+    replayed in real time against the continuous-batching server (HBM budget
+    200 GB, 3 launch slots) and against a plain one-square-at-a-time loop."""
+    import numpy as np
+    sides_all = [64, 128, 256, 512, 1024]
+    pool = {}
+    for sd in sides_all:
+        pool[sd] = []
+        for j in range(12):
+            t_in = torch.empty((sd, 128 * sd), dtype=torch.int32, device="cuda")
+            zoda_hip.fill_uniform(t_in, seed=seed + 100 * sd + j)
+            t_out = torch.empty((2 * sd, 128 * sd), dtype=torch.int32, device="cuda")
+            pool[sd].append((t_in, t_out))
+    torch.cuda.synchronize()
+    out = {}
+    for rate in rates:
+        rng = np.random.default_rng(seed + rate)
+        sides = rng.choice(sides_all, size=count).tolist()
+        gaps = rng.exponential(1e6 / rate, size=count)
+        arrivals = np.cumsum(gaps) - gaps[0]
+        specs = []
+        for i, sd in enumerate(sides):
+            t_in, t_out = pool[sd][i % len(pool[sd])]
+            specs.append((t_in.data_ptr(), t_out.data_ptr(), sd, 128 * sd, sd))
+        payload = sum(sd * sd * SHARE_BYTES for sd in sides)
+        res = {}
+        for mode, name in ((0, "served"), (1, "plain_loop")):
+            lat, st = zoda_hip.serve_trace(specs, arrivals, hbm_budget=200 * 10**9, slots=3, mode=mode)
+            res[name] = {
+                "p50_us": round(float(np.percentile(lat, 50)), 1),
+                "p99_us": round(float(np.percentile(lat, 99)), 1),
+                "mean_us": round(float(lat.mean()), 1),
+                "gbps": round(payload * 8 / 1e9 / (st["makespan_us"] / 1e6), 2),
+                "launches": st["batches"],
+                "max_batch": st["max_batch"],
+            }
+        out[f"{rate}_req_per_s"] = res
+    return {
+        "workload": f"{count} requests, Poisson arrivals, sides log-uniform {{64..1024}}, 512 B shares, "
+                    "HBM budget 200 GB, 3 slots; Gb/s = payload over first arrival to last completion",
+        **out,
     }
 
 
@@ -208,6 +321,15 @@ def main():
         except Exception as ex:  # report, do not fail the device-resident line
             e2e = {"error": str(ex)}
         result["end_to_end_pinned"] = e2e
+
+    del t_in, t_out
+    torch.cuda.empty_cache()
+    if not args.no_batched:
+        result["batched"] = batched_vs_plain(zoda_hip, torch, args.seed)
+        torch.cuda.empty_cache()
+    if not args.no_served:
+        result["served"] = served_vs_plain(zoda_hip, torch, args.seed, args.served_requests)
+        torch.cuda.empty_cache()
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(side, args.cpu_cols, args.seed)

@@ -142,6 +142,41 @@ int zoda_hip_autotune(const uint32_t* sides, uint32_t count, int iters, float* t
 int zoda_hip_get_variant(uint32_t log_side, int* variant);
 int zoda_hip_set_variant(uint32_t log_side, int variant);
 
+/* ---- serving (no counterpart in the reference) ----------------------------- */
+
+typedef struct zoda_serve_stats {
+  double makespan_us;  /* first arrival to last completion */
+  uint32_t batches;    /* launches: batched launches (server) or requests (plain loop) */
+  uint32_t max_batch;  /* most requests in one launch */
+  uint64_t peak_bytes; /* largest HBM footprint admitted at once */
+  uint32_t rejected;   /* requests larger than the whole budget */
+} zoda_serve_stats;
+
+/* Continuous-batching encode server on the current device.  Requests are
+ * device-resident squares (zoda_square).  Each of `slots` launch slots
+ * (1..16) owns a stream; whenever one is free, every queued request whose
+ * in+out bytes fit `hbm_budget` next to the requests still on the GPU goes out
+ * as one persistent batched launch, overlapping the launches in flight.  A
+ * scheduler thread runs until zoda_server_destroy.  A request larger than the
+ * whole budget is rejected with ZODA_EBUDGET (its ticket reads state 3). */
+typedef struct zoda_server zoda_server;
+int zoda_server_create(uint64_t hbm_budget, uint32_t slots, zoda_server** server);
+int zoda_server_submit(zoda_server* server, const zoda_square* request, uint64_t* ticket);
+/* state: 0 queued, 1 on the GPU, 2 done (latency_us = completion - arrival), 3 rejected */
+int zoda_server_poll(zoda_server* server, uint64_t ticket, int* state, double* latency_us);
+int zoda_server_wait_all(zoda_server* server);
+int zoda_server_stats(zoda_server* server, zoda_serve_stats* stats);
+int zoda_server_destroy(zoda_server* server);
+
+/* Replays a request trace in real time on this thread: request i arrives
+ * arrival_us[i] microseconds after the start (non-decreasing).  mode 0 runs
+ * the continuous-batching server above; mode 1 the plain loop, one
+ * zoda_hip_encode_device per request in arrival order on one stream.
+ * latency_us[i] receives completion minus arrival (-1 if rejected). */
+int zoda_hip_serve_trace(const zoda_square* requests, const double* arrival_us, uint32_t count,
+                         uint64_t hbm_budget, uint32_t slots, int mode, double* latency_us,
+                         zoda_serve_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -112,10 +112,17 @@ def test_autotune_variants_agree(zh, torch):
 
 
 def test_batch_argument_errors(zh, torch):
-    t = torch.empty((8, 8), dtype=torch.int32, device="cuda")
+    # every call here is rejected on the host before any launch; the buffers
+    # are still sized for the shapes named, so nothing could write past them
+    t_in = torch.empty((8192, 8), dtype=torch.int32, device="cuda")
+    t_out = torch.empty((16384, 8), dtype=torch.int32, device="cuda")
     with pytest.raises(zh.ZodaError):
-        zh.encode_batch([(t.data_ptr(), t.data_ptr(), 8192, 8, 8192)])   # side above 4096
+        zh.encode_batch([(t_in.data_ptr(), t_out.data_ptr(), 8192, 8, 8192)])  # side above 4096
     with pytest.raises(zh.ZodaError):
-        zh.encode_batch([(t.data_ptr(), t.data_ptr(), 8, 8, 1)])         # n_kn != 2 n_k
+        zh.encode_batch([(t_in.data_ptr(), t_out.data_ptr(), 8, 8, 0)])        # n_kn = n_k
+    with pytest.raises(zh.ZodaError):
+        zh.encode_batch([(t_in.data_ptr(), t_out.data_ptr(), 8, 8, 9)])        # n_kn = 4 n_k
+    with pytest.raises(zh.ZodaError):
+        zh.encode_batch([(t_in.data_ptr(), t_out.data_ptr(), 0, 8, 8)])        # k = 0
     with pytest.raises(zh.ZodaError):
         zh.encode_batch([])

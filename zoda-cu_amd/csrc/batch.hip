@@ -9,25 +9,14 @@
 // and replayed (zoda_hip_batch_plan_*).  The plan variant of each side comes
 // from the per-device autotune table (zoda_hip_autotune).
 #include <algorithm>
+#include <memory>
 #include <vector>
 
-#include "common.hpp"
+#include "batch.hpp"
 #include "encode_plans.hpp"
 #include "encode_tables.hpp"
 
 namespace zoda {
-
-struct SquareDesc {
-  const uint32_t* in;
-  uint32_t* out;
-  uint32_t k, cols, L, variant, full, tiles;
-};
-
-struct BatchItem {
-  uint32_t square, tile;
-};
-
-constexpr int MAX_LOG = 16;
 
 // Not inlined: each plan is register-allocated on its own, so the kernel's
 // VGPR budget is the largest plan's rather than a merge of all of them.
@@ -78,6 +67,62 @@ __global__ __launch_bounds__(BATCH_NT) void batch_kernel(const SquareDesc* __res
   }
 }
 
+int batch_grid(DeviceState* st, uint32_t* grid) {
+  int per_cu = 0, cus = 0;
+  ZODA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel, BATCH_NT,
+                                                         BATCH_LDS_WORDS * 4));
+  ZODA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, st->device));
+  *grid = (uint32_t)std::max(1, per_cu * cus);
+  return 0;
+}
+
+int launch_batch_items(const SquareDesc* d_squares, const BatchItem* d_items, uint32_t nitems,
+                       uint32_t* d_counter, const EncodeTabs* d_tabs, uint32_t grid,
+                       hipStream_t stream) {
+  hipLaunchKernelGGL(batch_reset, dim3(1), dim3(1), 0, stream, d_counter);
+  ZODA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(batch_kernel, dim3(std::min(grid, nitems)), dim3(BATCH_NT), BATCH_LDS_WORDS * 4,
+                     stream, d_squares, d_items, nitems, d_counter, d_tabs);
+  ZODA_HIP(hipGetLastError());
+  return 0;
+}
+
+int describe_square(DeviceState* st, const zoda_square& s, SquareDesc* d, EncodeTabs* tabs_by_slot,
+                    bool* have_slot) {
+  d->in = s.in;
+  d->out = s.out;
+  d->k = s.k;
+  d->cols = s.cols;
+  d->L = log2_ceil(s.k);
+  d->variant = (uint32_t)st->batch_variant[d->L];
+  return with_batch_plan((int)d->L, (int)d->variant, [&](auto plan) -> int {
+    using PL = decltype(plan);
+    d->tiles = (s.cols + PL::W - 1) / PL::W;
+    d->full = (s.cols % PL::W == 0 && s.k == (uint32_t)PL::S) ? 1u : 0u;
+    const size_t slot = (size_t)d->L * BATCH_VARIANTS + d->variant;
+    if (!have_slot[slot]) {
+      const EncodeTableSet* set = nullptr;
+      ZODA_TRY(get_encode_tables(st, PL::L, PL::R1, PL::R2, PL::RD, &set));
+      tabs_by_slot[slot] = EncodeTabs{set->base + set->inv1, set->base + set->inv2,
+                                      set->base + set->fwd1, set->base + set->fwd2,
+                                      set->base + set->twist, set->base + set->roots};
+      have_slot[slot] = true;
+    }
+    return 0;
+  });
+}
+
+int check_batch_square(const zoda_square& s, uint32_t i) {
+  if (!s.in || !s.out) return fail(ZODA_EINVAL, "square %u: NULL buffer", i);
+  if (s.k == 0 || s.cols == 0) return fail(ZODA_EINVAL, "square %u: k and cols must be positive", i);
+  const uint32_t L = log2_ceil(s.k);
+  if ((int)L > BATCH_MAX_LOG)
+    return fail(ZODA_EINVAL, "square %u: k = %u exceeds the batched maximum 2^%d", i, s.k, BATCH_MAX_LOG);
+  if (log2_ceil((uint64_t)s.k + s.n) != L + 1)
+    return fail(ZODA_EINVAL, "square %u: next_pow2(k + n) must be 2 * next_pow2(k)", i);
+  return 0;
+}
+
 }  // namespace zoda
 
 using namespace zoda;
@@ -104,24 +149,9 @@ static void free_plan(zoda_batch_plan* p) {
   delete p;
 }
 
-static int check_square(const zoda_square& s, uint32_t i) {
-  if (!s.in || !s.out) return fail(ZODA_EINVAL, "square %u: NULL buffer", i);
-  if (s.k == 0 || s.cols == 0) return fail(ZODA_EINVAL, "square %u: k and cols must be positive", i);
-  const uint32_t L = log2_ceil(s.k);
-  if ((int)L > BATCH_MAX_LOG)
-    return fail(ZODA_EINVAL, "square %u: k = %u exceeds the batched maximum 2^%d", i, s.k, BATCH_MAX_LOG);
-  if (log2_ceil((uint64_t)s.k + s.n) != L + 1)
-    return fail(ZODA_EINVAL, "square %u: next_pow2(k + n) must be 2 * next_pow2(k)", i);
-  return 0;
-}
-
 static int launch_batch_direct(zoda_batch_plan* p, hipStream_t stream) {
-  hipLaunchKernelGGL(batch_reset, dim3(1), dim3(1), 0, stream, p->d_counter);
-  ZODA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(batch_kernel, dim3(p->grid), dim3(BATCH_NT), BATCH_LDS_WORDS * 4, stream,
-                     p->d_squares, p->d_items, p->nitems, p->d_counter, p->d_tabs);
-  ZODA_HIP(hipGetLastError());
-  return 0;
+  return launch_batch_items(p->d_squares, p->d_items, p->nitems, p->d_counter, p->d_tabs, p->grid,
+                            stream);
 }
 
 // Builds descriptors, the largest-first item list and the per-(side, variant)
@@ -131,40 +161,18 @@ static int build_plan(const zoda_square* squares, uint32_t count, bool graph, zo
   if (!out) return fail(ZODA_EINVAL, "plan pointer is NULL");
   *out = nullptr;
   if (!squares || count == 0) return fail(ZODA_EINVAL, "no squares");
-  for (uint32_t i = 0; i < count; i++) ZODA_TRY(check_square(squares[i], i));
+  for (uint32_t i = 0; i < count; i++) ZODA_TRY(check_batch_square(squares[i], i));
   DeviceState* st = nullptr;
   ZODA_TRY(current_state(&st));
   std::lock_guard<std::recursive_mutex> lock(st->mu);
 
   std::vector<SquareDesc> descs(count);
   std::vector<EncodeTabs> tabs((BATCH_MAX_LOG + 1) * BATCH_VARIANTS, EncodeTabs{});
-  std::vector<bool> have(tabs.size(), false);
+  std::unique_ptr<bool[]> have(new bool[tabs.size()]());
   uint64_t nitems = 0;
   for (uint32_t i = 0; i < count; i++) {
-    const zoda_square& s = squares[i];
-    SquareDesc& d = descs[i];
-    d.in = s.in;
-    d.out = s.out;
-    d.k = s.k;
-    d.cols = s.cols;
-    d.L = log2_ceil(s.k);
-    d.variant = (uint32_t)st->batch_variant[d.L];
-    int rc = with_batch_plan((int)d.L, (int)d.variant, [&](auto plan) -> int {
-      using PL = decltype(plan);
-      d.tiles = (s.cols + PL::W - 1) / PL::W;
-      d.full = (s.cols % PL::W == 0 && s.k == (uint32_t)PL::S) ? 1u : 0u;
-      const size_t slot = (size_t)d.L * BATCH_VARIANTS + d.variant;
-      if (!have[slot]) {
-        const EncodeTableSet* set = nullptr;
-        ZODA_TRY(get_encode_tables(st, PL::L, PL::R1, PL::R2, PL::RD, &set));
-        tabs[slot] = EncodeTabs{set->base + set->inv1, set->base + set->inv2, set->base + set->fwd1,
-                                set->base + set->fwd2, set->base + set->twist, set->base + set->roots};
-        have[slot] = true;
-      }
-      return 0;
-    });
-    if (rc) return rc;
-    nitems += d.tiles;
+    ZODA_TRY(describe_square(st, squares[i], &descs[i], tabs.data(), have.get()));
+    nitems += descs[i].tiles;
   }
   if (nitems >= (1ull << 32)) return fail(ZODA_EINVAL, "batch of %llu tiles is too large", (unsigned long long)nitems);
   // Largest sides first so the long tiles start early and the short ones fill the tail.
@@ -196,11 +204,8 @@ static int build_plan(const zoda_square* squares, uint32_t count, bool graph, zo
   if (e == hipSuccess) e = hipMemcpy(p->d_tabs, tabs.data(), tabs.size() * sizeof(EncodeTabs), hipMemcpyHostToDevice);
   if (e != hipSuccess) return bail(fail((int)e, "batch plan upload failed: %s", hipGetErrorString(e)));
 
-  int per_cu = 0, cus = 0;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel, BATCH_NT, BATCH_LDS_WORDS * 4);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, st->device);
-  if (e != hipSuccess) return bail(fail((int)e, "occupancy query failed: %s", hipGetErrorString(e)));
-  p->grid = (uint32_t)std::max(1, per_cu * cus);
+  int rc = batch_grid(st, &p->grid);
+  if (rc) return bail(rc);
   p->grid = std::min<uint32_t>(p->grid, p->nitems);
 
   if (graph) {
@@ -208,7 +213,7 @@ static int build_plan(const zoda_square* squares, uint32_t count, bool graph, zo
     e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(fail((int)e, "stream creation failed: %s", hipGetErrorString(e)));
     e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-    int rc = e == hipSuccess ? launch_batch_direct(p, cap) : fail((int)e, "begin capture failed");
+    rc = e == hipSuccess ? launch_batch_direct(p, cap) : fail((int)e, "begin capture failed");
     hipError_t e2 = hipStreamEndCapture(cap, &p->graph);
     (void)hipStreamDestroy(cap);
     if (rc) return bail(rc);

@@ -1,0 +1,451 @@
+// Serving loop: continuous batching of encode requests on one device.
+//
+// Requests (device-resident squares, as zoda_square) queue in arrival order.
+// Each of `slots` launch slots owns a stream, a completion event and
+// preallocated device work-queue arrays.  Whenever a slot is free, every
+// queued request whose in+out bytes fit the HBM budget -- next to the
+// requests still on the GPU -- is admitted and the whole group goes out as one
+// persistent batched launch on that slot's stream.  Requests that arrive while
+// other batches run are admitted as soon as a slot and memory free up, and
+// their launch overlaps the ones in flight.  Twiddle tables stay resident in
+// the per-device cache.  Completion times come from the slot's event,
+// measured on the GPU against an event recorded at start.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "batch.hpp"
+#include "encode_plans.hpp"
+
+namespace zoda {
+
+namespace {
+
+constexpr uint32_t SERVE_MAX_BATCH = 64;
+constexpr uint32_t SERVE_MAX_ITEMS = 1u << 20;
+
+double now_us() {
+  using namespace std::chrono;
+  return duration<double, std::micro>(steady_clock::now().time_since_epoch()).count();
+}
+
+uint64_t request_bytes(const zoda_square& s) {
+  const uint64_t n_kn = 1ull << log2_ceil((uint64_t)s.k + s.n);
+  return ((uint64_t)s.k + n_kn) * s.cols * 4;
+}
+
+struct Request {
+  zoda_square sq;
+  double arrival_us = 0;   // host clock
+  double done_us = -1;     // host clock, from the GPU event
+  uint64_t bytes = 0;
+  int state = 0;           // 0 queued, 1 on the GPU, 2 done, 3 rejected
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  SquareDesc* d_squares = nullptr;
+  BatchItem* d_items = nullptr;
+  uint32_t* d_counter = nullptr;
+  SquareDesc* h_squares = nullptr;  // pinned staging
+  BatchItem* h_items = nullptr;
+  std::vector<uint64_t> members;    // request ids in this launch
+  uint64_t bytes = 0;
+  bool busy = false;
+};
+
+}  // namespace
+
+class Server {
+ public:
+  ~Server() { release(); }
+
+  int init(uint64_t budget, uint32_t nslots) {
+    ZODA_TRY(current_state(&st_));
+    budget_ = budget;
+    ZODA_TRY(batch_grid(st_, &grid_));
+    tabs_.assign((BATCH_MAX_LOG + 1) * BATCH_VARIANTS, EncodeTabs{});
+    have_.reset(new bool[tabs_.size()]());
+    ZODA_HIP(hipMalloc((void**)&d_tabs_, tabs_.size() * sizeof(EncodeTabs)));
+    ZODA_HIP(hipEventCreate(&start_));
+    slots_.resize(nslots);
+    for (Slot& s : slots_) {
+      ZODA_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+      ZODA_HIP(hipEventCreate(&s.done));
+      if (hipMalloc((void**)&s.d_squares, SERVE_MAX_BATCH * sizeof(SquareDesc)) != hipSuccess ||
+          hipMalloc((void**)&s.d_items, SERVE_MAX_ITEMS * sizeof(BatchItem)) != hipSuccess ||
+          hipMalloc((void**)&s.d_counter, sizeof(uint32_t)) != hipSuccess ||
+          hipHostMalloc((void**)&s.h_squares, SERVE_MAX_BATCH * sizeof(SquareDesc), hipHostMallocDefault) != hipSuccess ||
+          hipHostMalloc((void**)&s.h_items, SERVE_MAX_ITEMS * sizeof(BatchItem), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ZODA_ENOMEM, "serving slot allocation failed");
+      }
+    }
+    // Reference point of the GPU clock: the host time right after `start_` completes.
+    ZODA_HIP(hipEventRecord(start_, slots_[0].stream));
+    ZODA_HIP(hipEventSynchronize(start_));
+    start_host_us_ = now_us();
+    return 0;
+  }
+
+  // Thread-safe enqueue; returns the request id.
+  int submit(const zoda_square& sq, double arrival_us, uint64_t* id) {
+    ZODA_TRY(check_batch_square(sq, 0));
+    std::lock_guard<std::mutex> lock(mu_);
+    Request r;
+    r.sq = sq;
+    r.arrival_us = arrival_us;
+    r.bytes = request_bytes(sq);
+    if (r.bytes > budget_) r.state = 3;
+    reqs_.push_back(r);
+    *id = reqs_.size() - 1;
+    if (r.state == 0) queue_.push_back(*id);
+    else rejected_++;
+    return 0;
+  }
+
+  // One scheduler step: retire finished launches, then fill free slots.
+  int pump() {
+    std::lock_guard<std::mutex> lock(mu_);
+    for (Slot& s : slots_) {
+      if (!s.busy) continue;
+      hipError_t q = hipEventQuery(s.done);
+      if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) return fail((int)q, "serving launch failed: %s", hipGetErrorString(q));
+      float ms = 0;
+      ZODA_HIP(hipEventElapsedTime(&ms, start_, s.done));
+      const double t = start_host_us_ + ms * 1000.0;
+      for (uint64_t id : s.members) {
+        reqs_[id].done_us = t;
+        reqs_[id].state = 2;
+        completed_++;
+      }
+      in_flight_bytes_ -= s.bytes;
+      s.members.clear();
+      s.bytes = 0;
+      s.busy = false;
+    }
+    for (Slot& s : slots_) {
+      if (s.busy || queue_.empty()) continue;
+      ZODA_TRY(launch_slot(s));
+    }
+    return 0;
+  }
+
+  bool idle() {
+    std::lock_guard<std::mutex> lock(mu_);
+    if (!queue_.empty()) return false;
+    for (Slot& s : slots_)
+      if (s.busy) return false;
+    return true;
+  }
+
+  int poll(uint64_t id, int* state, double* latency_us) {
+    std::lock_guard<std::mutex> lock(mu_);
+    if (id >= reqs_.size()) return fail(ZODA_EINVAL, "unknown request %llu", (unsigned long long)id);
+    const Request& r = reqs_[id];
+    *state = r.state;
+    *latency_us = r.state == 2 ? r.done_us - r.arrival_us : -1.0;
+    return 0;
+  }
+
+  void fill_stats(zoda_serve_stats* s) {
+    std::lock_guard<std::mutex> lock(mu_);
+    s->batches = batches_;
+    s->max_batch = max_batch_;
+    s->peak_bytes = peak_bytes_;
+    s->rejected = rejected_;
+    double first = 1e300, last = 0;
+    for (const Request& r : reqs_) {
+      first = std::min(first, r.arrival_us);
+      if (r.state == 2) last = std::max(last, r.done_us);
+    }
+    s->makespan_us = last > first ? last - first : 0;
+  }
+
+  double latency(uint64_t id) {
+    std::lock_guard<std::mutex> lock(mu_);
+    const Request& r = reqs_[id];
+    return r.state == 2 ? r.done_us - r.arrival_us : -1.0;
+  }
+
+  // Background scheduler for the online API.
+  void start_thread() {
+    stop_ = false;
+    thread_ = std::thread([this] {
+      while (!stop_.load()) {
+        if (pump() != 0) {
+          std::lock_guard<std::mutex> lock(mu_);
+          thread_error_ = true;
+          break;
+        }
+        std::this_thread::yield();
+      }
+    });
+  }
+
+  void stop_thread() {
+    if (thread_.joinable()) {
+      stop_ = true;
+      thread_.join();
+    }
+  }
+
+  bool thread_error() {
+    std::lock_guard<std::mutex> lock(mu_);
+    return thread_error_;
+  }
+
+ private:
+  int launch_slot(Slot& s) {
+    uint32_t nsq = 0, nitems = 0;
+    uint64_t bytes = 0;
+    std::vector<uint64_t> taken;
+    // FIFO admission: stop at the first request that does not fit.
+    while (!queue_.empty() && nsq < SERVE_MAX_BATCH) {
+      const uint64_t id = queue_.front();
+      Request& r = reqs_[id];
+      if (in_flight_bytes_ + bytes + r.bytes > budget_) break;
+      SquareDesc d;
+      ZODA_TRY(describe_square(st_, r.sq, &d, tabs_.data(), have_.get()));
+      if (nitems + d.tiles > SERVE_MAX_ITEMS) break;
+      s.h_squares[nsq] = d;
+      nitems += d.tiles;
+      bytes += r.bytes;
+      taken.push_back(id);
+      queue_.pop_front();
+      nsq++;
+    }
+    if (nsq == 0) return 0;
+    // Largest sides first inside the launch.
+    std::vector<uint32_t> order(nsq);
+    for (uint32_t i = 0; i < nsq; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return s.h_squares[a].L > s.h_squares[b].L; });
+    uint32_t n = 0;
+    for (uint32_t i : order)
+      for (uint32_t t = 0; t < s.h_squares[i].tiles; t++) s.h_items[n++] = BatchItem{i, t};
+    if (tabs_dirty()) ZODA_HIP(hipMemcpyAsync(d_tabs_, tabs_.data(), tabs_.size() * sizeof(EncodeTabs),
+                                              hipMemcpyHostToDevice, s.stream));
+    ZODA_HIP(hipMemcpyAsync(s.d_squares, s.h_squares, nsq * sizeof(SquareDesc), hipMemcpyHostToDevice, s.stream));
+    ZODA_HIP(hipMemcpyAsync(s.d_items, s.h_items, n * sizeof(BatchItem), hipMemcpyHostToDevice, s.stream));
+    ZODA_TRY(launch_batch_items(s.d_squares, s.d_items, n, s.d_counter, d_tabs_, grid_, s.stream));
+    ZODA_HIP(hipEventRecord(s.done, s.stream));
+    for (uint64_t id : taken) reqs_[id].state = 1;
+    s.members = std::move(taken);
+    s.bytes = bytes;
+    s.busy = true;
+    in_flight_bytes_ += bytes;
+    peak_bytes_ = std::max(peak_bytes_, in_flight_bytes_);
+    batches_++;
+    max_batch_ = std::max(max_batch_, nsq);
+    return 0;
+  }
+
+  // The table pointer array only grows; upload it when a new (side, variant)
+  // appeared.  Pinned staging is not needed: it changes rarely and the copy
+  // is stream-ordered before the launch that reads it.
+  bool tabs_dirty() {
+    size_t n = 0;
+    for (size_t i = 0; i < tabs_.size(); i++) n += have_[i];
+    if (n == tabs_uploaded_) return false;
+    tabs_uploaded_ = n;
+    return true;
+  }
+
+  void release() {
+    stop_thread();
+    for (Slot& s : slots_) {
+      if (s.stream) (void)hipStreamSynchronize(s.stream);
+      if (s.done) (void)hipEventDestroy(s.done);
+      if (s.stream) (void)hipStreamDestroy(s.stream);
+      (void)hipFree(s.d_squares);
+      (void)hipFree(s.d_items);
+      (void)hipFree(s.d_counter);
+      if (s.h_squares) (void)hipHostFree(s.h_squares);
+      if (s.h_items) (void)hipHostFree(s.h_items);
+    }
+    slots_.clear();
+    if (start_) (void)hipEventDestroy(start_);
+    start_ = nullptr;
+    (void)hipFree(d_tabs_);
+    d_tabs_ = nullptr;
+  }
+
+  DeviceState* st_ = nullptr;
+  uint64_t budget_ = 0, in_flight_bytes_ = 0, peak_bytes_ = 0;
+  uint32_t grid_ = 0, batches_ = 0, max_batch_ = 0, rejected_ = 0;
+  uint64_t completed_ = 0;
+  std::vector<EncodeTabs> tabs_;
+  std::unique_ptr<bool[]> have_;
+  size_t tabs_uploaded_ = 0;
+  EncodeTabs* d_tabs_ = nullptr;
+  hipEvent_t start_ = nullptr;
+  double start_host_us_ = 0;
+  std::vector<Slot> slots_;
+  std::vector<Request> reqs_;
+  std::deque<uint64_t> queue_;
+  std::mutex mu_;
+  std::thread thread_;
+  std::atomic<bool> stop_{false};
+  bool thread_error_ = false;
+};
+
+}  // namespace zoda
+
+using namespace zoda;
+
+struct zoda_server {
+  Server impl;
+};
+
+extern "C" int zoda_server_create(uint64_t hbm_budget, uint32_t slots, zoda_server** out) {
+  clear_error();
+  if (!out || slots == 0 || slots > 16 || hbm_budget == 0) return fail(ZODA_EINVAL, "bad arguments");
+  *out = nullptr;
+  zoda_server* s = new zoda_server();
+  int rc = s->impl.init(hbm_budget, slots);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  s->impl.start_thread();
+  *out = s;
+  return 0;
+}
+
+extern "C" int zoda_server_submit(zoda_server* srv, const zoda_square* req, uint64_t* ticket) {
+  clear_error();
+  if (!srv || !req || !ticket) return fail(ZODA_EINVAL, "bad arguments");
+  if (srv->impl.thread_error()) return fail(ZODA_ESTATE, "the scheduler stopped after an error");
+  const int rc = srv->impl.submit(*req, now_us(), ticket);
+  if (rc) return rc;
+  if (request_bytes(*req) > 0 && srv->impl.latency(*ticket) < 0) {
+    int state = 0;
+    double lat = 0;
+    srv->impl.poll(*ticket, &state, &lat);
+    if (state == 3) return fail(ZODA_EBUDGET, "request of %llu bytes exceeds the HBM budget",
+                                (unsigned long long)request_bytes(*req));
+  }
+  return 0;
+}
+
+// state: 0 queued, 1 on the GPU, 2 done, 3 rejected; latency_us valid when done.
+extern "C" int zoda_server_poll(zoda_server* srv, uint64_t ticket, int* state, double* latency_us) {
+  clear_error();
+  if (!srv || !state || !latency_us) return fail(ZODA_EINVAL, "bad arguments");
+  return srv->impl.poll(ticket, state, latency_us);
+}
+
+extern "C" int zoda_server_wait_all(zoda_server* srv) {
+  clear_error();
+  if (!srv) return fail(ZODA_EINVAL, "bad arguments");
+  while (!srv->impl.idle()) {
+    if (srv->impl.thread_error()) return fail(ZODA_ESTATE, "the scheduler stopped after an error");
+    std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+  return 0;
+}
+
+extern "C" int zoda_server_stats(zoda_server* srv, zoda_serve_stats* stats) {
+  clear_error();
+  if (!srv || !stats) return fail(ZODA_EINVAL, "bad arguments");
+  srv->impl.fill_stats(stats);
+  return 0;
+}
+
+extern "C" int zoda_server_destroy(zoda_server* srv) {
+  clear_error();
+  if (srv) {
+    srv->impl.stop_thread();
+    delete srv;
+  }
+  return 0;
+}
+
+// Trace replay: requests arrive at arrival_us[i] (microseconds after the
+// start, non-decreasing); this thread plays both the arrival process and the
+// scheduler.  mode 0 is the continuous-batching server above; mode 1 is the
+// plain loop, one single-square encode per request in arrival order on one
+// stream.
+extern "C" int zoda_hip_serve_trace(const zoda_square* reqs, const double* arrival_us, uint32_t count,
+                                    uint64_t hbm_budget, uint32_t slots, int mode,
+                                    double* latency_us, zoda_serve_stats* stats) {
+  clear_error();
+  if (!reqs || !arrival_us || !latency_us || count == 0 || (mode != 0 && mode != 1))
+    return fail(ZODA_EINVAL, "bad arguments");
+  for (uint32_t i = 0; i < count; i++) {
+    ZODA_TRY(check_batch_square(reqs[i], i));
+    if (i && arrival_us[i] < arrival_us[i - 1]) return fail(ZODA_EINVAL, "arrivals must be sorted");
+  }
+  if (mode == 0) {
+    if (slots == 0 || slots > 16) return fail(ZODA_EINVAL, "slots must be 1..16");
+    Server srv;
+    ZODA_TRY(srv.init(hbm_budget, slots));
+    const double t0 = now_us();
+    uint32_t next = 0;
+    std::vector<uint64_t> ids(count);
+    while (next < count || !srv.idle()) {
+      const double t = now_us() - t0;
+      while (next < count && arrival_us[next] <= t) {
+        ZODA_TRY(srv.submit(reqs[next], t0 + arrival_us[next], &ids[next]));
+        next++;
+      }
+      ZODA_TRY(srv.pump());
+    }
+    for (uint32_t i = 0; i < count; i++) latency_us[i] = srv.latency(ids[i]);
+    if (stats) srv.fill_stats(stats);
+    return 0;
+  }
+  // plain loop
+  DeviceState* st = nullptr;
+  ZODA_TRY(current_state(&st));
+  hipStream_t s;
+  ZODA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  std::vector<hipEvent_t> ev(count + 1, nullptr);
+  int rc = 0;
+  for (auto& e : ev)
+    if (!rc && hipEventCreate(&e) != hipSuccess) rc = fail(ZODA_ENOMEM, "event creation failed");
+  double start_host = 0, t0 = 0;
+  if (!rc) {
+    hipEventRecord(ev[count], s);
+    hipEventSynchronize(ev[count]);
+    start_host = now_us();
+    t0 = start_host;
+  }
+  uint64_t peak = 0;
+  for (uint32_t i = 0; i < count && !rc; i++) {
+    while (now_us() - t0 < arrival_us[i]) {
+    }
+    const zoda_square& q = reqs[i];
+    rc = launch_encode(st, q.in, q.k, q.cols, q.n, q.out, ZODA_LAYOUT_ROW, s);
+    if (!rc && hipEventRecord(ev[i], s) != hipSuccess) rc = fail(ZODA_ESTATE, "event record failed");
+    peak = std::max(peak, request_bytes(q));
+  }
+  if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(ZODA_ESTATE, "plain loop failed");
+  double last = 0;
+  for (uint32_t i = 0; i < count && !rc; i++) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, ev[count], ev[i]);
+    const double done = start_host + ms * 1000.0;
+    latency_us[i] = done - (t0 + arrival_us[i]);
+    last = std::max(last, done);
+  }
+  if (!rc && stats) {
+    stats->makespan_us = last - (t0 + arrival_us[0]);
+    stats->batches = count;
+    stats->max_batch = 1;
+    stats->peak_bytes = peak;
+    stats->rejected = 0;
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(s);
+  return rc;
+}

@@ -76,6 +76,15 @@ def _load():
         "zoda_hip_autotune": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_int, _vp]),
         "zoda_hip_get_variant": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(ctypes.c_int)]),
         "zoda_hip_set_variant": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int]),
+        "zoda_server_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_vp)]),
+        "zoda_server_submit": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "zoda_server_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_double)]),
+        "zoda_server_wait_all": (ctypes.c_int, [_vp]),
+        "zoda_server_stats": (ctypes.c_int, [_vp, _vp]),
+        "zoda_server_destroy": (ctypes.c_int, [_vp]),
+        "zoda_hip_serve_trace": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
+                                                ctypes.c_int, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -353,3 +362,67 @@ def get_variant(side: int) -> int:
 
 def set_variant(side: int, variant: int) -> None:
     _check(lib.zoda_hip_set_variant(side.bit_length() - 1, variant), "zoda_hip_set_variant")
+
+
+# ---- serving --------------------------------------------------------------------------
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("makespan_us", ctypes.c_double), ("batches", ctypes.c_uint32),
+                ("max_batch", ctypes.c_uint32), ("peak_bytes", ctypes.c_uint64),
+                ("rejected", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def serve_trace(specs, arrival_us, hbm_budget: int, slots: int = 3, mode: int = 0):
+    """Replays a trace (specs as for encode_batch, arrivals in microseconds);
+    mode 0 = continuous batching, 1 = plain loop.  Returns (latencies_us, stats)."""
+    arr = _squares(specs)
+    arrivals = np.ascontiguousarray(arrival_us, dtype=np.float64)
+    lat = np.zeros(len(specs), dtype=np.float64)
+    st = ServeStats()
+    _check(lib.zoda_hip_serve_trace(ctypes.addressof(arr), arrivals.ctypes.data, len(specs), hbm_budget,
+                                    slots, mode, lat.ctypes.data, ctypes.addressof(st)),
+           "zoda_hip_serve_trace")
+    return lat, st.as_dict()
+
+
+class Server:
+    """Online continuous-batching server (background scheduler thread)."""
+
+    def __init__(self, hbm_budget: int, slots: int = 3):
+        self._srv = ctypes.c_void_p()
+        _check(lib.zoda_server_create(hbm_budget, slots, ctypes.byref(self._srv)), "zoda_server_create")
+
+    def submit(self, d_in: int, d_out: int, k: int, cols: int, n: int) -> int:
+        sq = Square(d_in, d_out, k, cols, n, 0)
+        t = ctypes.c_uint64(0)
+        _check(lib.zoda_server_submit(self._srv, ctypes.addressof(sq), ctypes.byref(t)), "zoda_server_submit")
+        return t.value
+
+    def poll(self, ticket: int):
+        state = ctypes.c_int(0)
+        lat = ctypes.c_double(0)
+        _check(lib.zoda_server_poll(self._srv, ticket, ctypes.byref(state), ctypes.byref(lat)),
+               "zoda_server_poll")
+        return state.value, lat.value
+
+    def wait_all(self) -> None:
+        _check(lib.zoda_server_wait_all(self._srv), "zoda_server_wait_all")
+
+    def stats(self) -> dict:
+        st = ServeStats()
+        _check(lib.zoda_server_stats(self._srv, ctypes.addressof(st)), "zoda_server_stats")
+        return st.as_dict()
+
+    def close(self) -> None:
+        if self._srv:
+            lib.zoda_server_destroy(self._srv)
+            self._srv = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
