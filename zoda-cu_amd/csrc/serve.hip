@@ -102,7 +102,9 @@ class Server {
     r.sq = sq;
     r.arrival_us = arrival_us;
     r.bytes = request_bytes(sq);
-    if (r.bytes > budget_) r.state = 3;
+    SquareDesc d;
+    ZODA_TRY(describe_square(st_, sq, &d, tabs_.data(), have_.get()));
+    if (r.bytes > budget_ || d.tiles > SERVE_MAX_ITEMS) r.state = 3;  // could never be admitted
     reqs_.push_back(r);
     *id = reqs_.size() - 1;
     if (r.state == 0) queue_.push_back(*id);
