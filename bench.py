@@ -41,7 +41,14 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the 96-square batched comparison")
     ap.add_argument("--no-served", action="store_true", help="skip the served-trace comparison")
     ap.add_argument("--served-requests", type=int, default=1000)
-    return ap.parse_args()
+    ap.add_argument("--only-encode", action="store_true",
+                    help="time the encode only (no end-to-end, batched, served or CPU legs); for profiling")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="instead of benchmarking, run 3 device copies of MIB MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.only_encode:
+        args.no_e2e = args.no_cpu = args.no_batched = args.no_served = True
+    return args
 
 
 def traffic_from_profile(side: int):
@@ -217,6 +224,16 @@ def main():
     import numpy as np
     import torch
     import zoda_hip
+
+    if args.copy_calibration:
+        n = args.copy_calibration << 18   # int32 elements in MIB MiB
+        src = torch.ones(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "copies": 3}))
+        return
 
     side = args.side
     cols = 128 * side
