@@ -92,8 +92,20 @@ def sim_lib():
         lib.zoda_sim_encode_variant.restype = ctypes.c_int
         lib.zoda_sim_encode_variant.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                                 u32p, ctypes.c_int, ctypes.c_int]
+        lib.zoda_sim_encode_persistent.restype = ctypes.c_int
+        lib.zoda_sim_encode_persistent.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p]
         _sim = lib
     return _sim
+
+
+def sim_encode_persistent(data: np.ndarray) -> np.ndarray:
+    k, cols = data.shape
+    out = np.zeros((2 * k, cols), dtype=np.uint32)
+    d = np.ascontiguousarray(data, dtype=np.uint32)
+    rc = sim_lib().zoda_sim_encode_persistent(d.ctypes.data, k, cols, out.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"no persistent plan for k={k}, cols={cols} (status {rc})")
+    return out
 
 
 def sim_encode(data: np.ndarray, n: int | None = None, layout: int = 0, variant: int = -1) -> np.ndarray:

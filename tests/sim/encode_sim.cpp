@@ -18,6 +18,8 @@ using namespace zoda;
 template <class PL, int STEP, bool FULL>
 static void run_steps(uint32_t tile, const uint32_t* in, uint32_t* out, uint32_t krows,
                       uint32_t cols, int layout, int32_t* lds, const EncodeTabs& T) {
+  if constexpr (STEP == 0)
+    for (int tid = 0; tid < PL::NT; tid++) encode_load_tables<PL>(tid, lds, T);
   for (int tid = 0; tid < PL::NT; tid++)
     encode_step<PL, STEP, FULL>(tid, tile, in, out, krows, cols, layout, lds, T);
   if constexpr (STEP + 1 < PL::STEPS)
@@ -28,8 +30,9 @@ template <class PL>
 static int sim_plan(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out, int layout) {
   EncodeTabOffsets off;
   std::vector<Tw> tabs = build_encode_tables(PL::L, PL::R1, PL::R2, PL::RD, &off);
-  EncodeTabs T{tabs.data() + off.inv1, tabs.data() + off.inv2, tabs.data() + off.fwd1,
-               tabs.data() + off.fwd2, tabs.data() + off.twist, tabs.data() + off.roots};
+  PackedTables pk = pack_u32(tabs, off);
+  EncodeTabs T = make_encode_tabs(tabs.data(), off, pk.words.data(), pk);
+  if (PL::TWL && pk.words.size() > (size_t)PL::TW_WORDS) return -2;
   std::vector<int32_t> lds(PL::LDS_WORDS ? PL::LDS_WORDS : 1);
   const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
   const bool full = cols % PL::W == 0 && k == (uint32_t)PL::S;
@@ -40,6 +43,49 @@ static int sim_plan(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out
       run_steps<PL, 0, false>(tile, in, out, k, cols, layout, lds.data(), T);
   }
   return 0;
+}
+
+template <class PL, int STEP>
+static void run_persistent_steps(uint32_t tile, const uint32_t* in, uint32_t* out, uint32_t cols,
+                                 int32_t* lds, const EncodeTabs& T) {
+  for (int tid = 0; tid < PL::NT; tid++)
+    encode_step<PL, STEP, true, true>(tid, tile, in, out, (uint32_t)PL::S, cols, 0, lds, T);
+  if constexpr (STEP + 1 < PL::STEPS) run_persistent_steps<PL, STEP + 1>(tile, in, out, cols, lds, T);
+}
+
+// The persistent kernel's order of work on one work-group: tables, prefetch of
+// the first tile, then per tile all passes (the last pass holds its values
+// before the next prefetch overwrites the tile, which is what running every
+// thread's pass to completion first models) and the next prefetch.
+template <class PL>
+static int sim_persistent(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out) {
+  if (cols % PL::W != 0 || k != (uint32_t)PL::S) return -3;
+  EncodeTabOffsets off;
+  std::vector<Tw> tabs = build_encode_tables(PL::L, PL::R1, PL::R2, PL::RD, &off);
+  PackedTables pk = pack_u32(tabs, off);
+  EncodeTabs T = make_encode_tabs(tabs.data(), off, pk.words.data(), pk);
+  if (pk.words.size() > (size_t)PL::TW_WORDS) return -2;
+  std::vector<int32_t> lds(PL::LDS_WORDS);
+  for (int tid = 0; tid < PL::NT; tid++) encode_load_tables<PL>(tid, lds.data(), T);
+  const uint32_t ntiles = cols / PL::W;
+  for (int tid = 0; tid < PL::NT; tid++) prefetch_tile<PL>(tid, 0, in, cols, lds.data());
+  for (uint32_t tile = 0; tile < ntiles; tile++) {
+    run_persistent_steps<PL, 0>(tile, in, out, cols, lds.data(), T);
+    if (tile + 1 < ntiles)
+      for (int tid = 0; tid < PL::NT; tid++) prefetch_tile<PL>(tid, tile + 1, in, cols, lds.data());
+  }
+  return 0;
+}
+
+// Persistent form of the single-square plan of side k (whole tiles only).
+extern "C" int zoda_sim_encode_persistent(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out) {
+  uint32_t nk = 1, L = 0;
+  while (nk < k) { nk <<= 1; L++; }
+  return with_encode_plan((int)L, [&](auto plan) -> int {
+    using PL = decltype(plan);
+    if constexpr (PL::PERSIST) return sim_persistent<PL>(in, k, cols, out);
+    else return -4;
+  });
 }
 
 // variant < 0: the single-square plan of the side; 0/1: the batched plans.
@@ -59,24 +105,5 @@ extern "C" int zoda_sim_encode_variant(const uint32_t* in, uint32_t k, uint32_t 
 
 extern "C" int zoda_sim_encode(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t n,
                                uint32_t* out, int layout) {
-  uint32_t nk = 1, L = 0;
-  while (nk < k) { nk <<= 1; L++; }
-  if (k == 0 || cols == 0 || k + n <= nk || k + n > 2 * nk) return -1;
-  return with_encode_plan((int)L, [&](auto plan) -> int {
-    using PL = decltype(plan);
-    EncodeTabOffsets off;
-    std::vector<Tw> tabs = build_encode_tables(PL::L, PL::R1, PL::R2, PL::RD, &off);
-    EncodeTabs T{tabs.data() + off.inv1, tabs.data() + off.inv2, tabs.data() + off.fwd1,
-                 tabs.data() + off.fwd2, tabs.data() + off.twist, tabs.data() + off.roots};
-    std::vector<int32_t> lds(PL::LDS_WORDS ? PL::LDS_WORDS : 1);
-    const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
-    const bool full = cols % PL::W == 0 && k == (uint32_t)PL::S;
-    for (uint32_t tile = 0; tile < ntiles; tile++) {
-      if (full)
-        run_steps<PL, 0, true>(tile, in, out, k, cols, layout, lds.data(), T);
-      else
-        run_steps<PL, 0, false>(tile, in, out, k, cols, layout, lds.data(), T);
-    }
-    return 0;
-  });
+  return zoda_sim_encode_variant(in, k, cols, n, out, layout, -1);
 }

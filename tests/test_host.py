@@ -123,3 +123,13 @@ def test_batched_plans_emulation(k, variant):
     cols = 70 if k <= 1024 else 12
     data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)
     assert np.array_equal(sim_encode(data, variant=variant), O.encode(data, threads=4))
+
+
+@pytest.mark.parametrize("k,cols", [(1024, 96), (2048, 48)])
+def test_persistent_kernel_emulation(k, cols):
+    # prefetch of each next tile into LDS, first pass reading it back, last
+    # pass holding its values across the prefetch
+    from helpers import sim_encode_persistent
+    rng = np.random.default_rng(k)
+    data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)
+    assert np.array_equal(sim_encode_persistent(data), O.encode(data, threads=4))

@@ -103,9 +103,7 @@ int describe_square(DeviceState* st, const zoda_square& s, SquareDesc* d, Encode
     if (!have_slot[slot]) {
       const EncodeTableSet* set = nullptr;
       ZODA_TRY(get_encode_tables(st, PL::L, PL::R1, PL::R2, PL::RD, &set));
-      tabs_by_slot[slot] = EncodeTabs{set->base + set->inv1, set->base + set->inv2,
-                                      set->base + set->fwd1, set->base + set->fwd2,
-                                      set->base + set->twist, set->base + set->roots};
+      tabs_by_slot[slot] = *set->tabs;
       have_slot[slot] = true;
     }
     return 0;

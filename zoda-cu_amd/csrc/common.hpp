@@ -50,9 +50,13 @@ struct MontTables {
   uint32_t* inv = nullptr;
 };
 
+struct EncodeTabs;
+// Device copy of the encode tables of one (side, digit split): the (w, w/p)
+// buffer, its packed u32 copy, and the EncodeTabs pointing into both.
 struct EncodeTableSet {
   Tw* base = nullptr;
-  size_t inv1 = 0, inv2 = 0, fwd1 = 0, fwd2 = 0, twist = 0, roots = 0;
+  uint32_t* packed = nullptr;
+  EncodeTabs* tabs = nullptr;  // host-side struct (owned)
 };
 
 struct DeviceState {
@@ -90,13 +94,11 @@ inline uint32_t encode_table_key(int L, int r1, int r2, int rd) {
 // Constant tables of the fused encode for side 2^L and digit split (r1, r2, rd).
 int get_encode_tables(DeviceState* st, int L, int r1, int r2, int rd, const EncodeTableSet** out);
 
-// Fused column encode on device memory (encode.hip).  in_pitch/out_pitch are
-// the row pitches (elements) of the row-major input and row-major output; for
-// the column layout out_pitch is ignored.
+// Fused column encode on device memory (encode.hip).
 int launch_encode(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols,
                   uint32_t n, uint32_t* d_out, int layout, hipStream_t stream);
 
-// Seeded uniform generator (fill.hip).
+// Seeded uniform generator (api.hip).
 int launch_fill_uniform(uint32_t* d_out, size_t count, uint64_t seed, hipStream_t stream);
 
 }  // namespace zoda

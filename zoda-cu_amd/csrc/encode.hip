@@ -17,14 +17,21 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-template <class PL, bool FULL>
+// LAYOUT is a template parameter so each variant addresses its output with
+// one base and a constant stride per value.
+template <class PL, bool FULL, int LAYOUT>
 __global__ __launch_bounds__(PL::NT) void encode_kernel(const uint32_t* __restrict__ in,
                                                         uint32_t* __restrict__ out,
                                                         uint32_t krows, uint32_t cols,
-                                                        int layout, EncodeTabs T) {
+                                                        EncodeTabs T) {
+  constexpr int layout = LAYOUT;
   extern __shared__ int32_t lds[];
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = (int)threadIdx.x;
+  if constexpr (PL::TWL) {
+    encode_load_tables<PL>(tid, lds, T);
+    __syncthreads();
+  }
   encode_step<PL, 0, FULL>(tid, tile, in, out, krows, cols, layout, lds, T);
   if constexpr (PL::STEPS > 1) {
     __syncthreads();
@@ -40,6 +47,43 @@ __global__ __launch_bounds__(PL::NT) void encode_kernel(const uint32_t* __restri
   }
 }
 
+// Persistent form for whole tiles: one work-group per CU walks its tiles
+// (XCD-contiguous, as above), keeps the twiddle tables in LDS for all of them,
+// and prefetches the next tile's input straight into LDS while the last pass
+// of the current tile computes and stores.
+template <class PL>
+__global__ __launch_bounds__(PL::NT) void encode_persistent(const uint32_t* __restrict__ in,
+                                                            uint32_t* __restrict__ out,
+                                                            uint32_t cols, uint32_t ntiles,
+                                                            EncodeTabs T) {
+  extern __shared__ int32_t lds[];
+  const int tid = (int)threadIdx.x;
+  encode_load_tables<PL>(tid, lds, T);
+  uint32_t l = blockIdx.x;
+  if (l < ntiles) prefetch_tile<PL>(tid, xcd_tile(l, ntiles), in, cols, lds);
+  for (; l < ntiles; l += gridDim.x) {
+    const uint32_t tile = xcd_tile(l, ntiles);
+    const uint32_t next = l + gridDim.x;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's prefetch (and tables) landed
+    __syncthreads();                // ... and every other wave's
+    const uint32_t k = (uint32_t)PL::S;
+    encode_step<PL, 0, true, true>(tid, tile, in, out, k, cols, 0, lds, T);
+    __syncthreads();
+    encode_step<PL, 1, true, true>(tid, tile, in, out, k, cols, 0, lds, T);
+    __syncthreads();
+    encode_step<PL, 2, true, true>(tid, tile, in, out, k, cols, 0, lds, T);
+    if constexpr (PL::STEPS > 3) {
+      __syncthreads();
+      encode_step<PL, 3, true, true>(tid, tile, in, out, k, cols, 0, lds, T);
+      __syncthreads();
+      encode_step<PL, 4, true, true>(tid, tile, in, out, k, cols, 0, lds, T, [&] {
+        __syncthreads();  // every wave holds its last-pass values: the tile is free
+        if (next < ntiles) prefetch_tile<PL>(tid, xcd_tile(next, ntiles), in, cols, lds);
+      });
+    }
+  }
+}
+
 int get_encode_tables(DeviceState* st, int L, int r1, int r2, int rd, const EncodeTableSet** out) {
   std::lock_guard<std::recursive_mutex> lock(st->mu);
   const uint32_t key = encode_table_key(L, r1, r2, rd);
@@ -47,19 +91,23 @@ int get_encode_tables(DeviceState* st, int L, int r1, int r2, int rd, const Enco
   if (it == st->encode.end()) {
     EncodeTabOffsets off;
     std::vector<Tw> host = build_encode_tables(L, r1, r2, rd, &off);
+    PackedTables pk = pack_u32(host, off);
     EncodeTableSet set;
-    ZODA_HIP(hipMalloc((void**)&set.base, host.size() * sizeof(Tw)));
+    if (hipMalloc((void**)&set.base, host.size() * sizeof(Tw)) != hipSuccess ||
+        hipMalloc((void**)&set.packed, pk.words.size() * sizeof(uint32_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(set.base);
+      return fail(ZODA_ENOMEM, "encode table allocation failed");
+    }
     hipError_t e = hipMemcpy(set.base, host.data(), host.size() * sizeof(Tw), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(set.packed, pk.words.data(), pk.words.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       (void)hipFree(set.base);
+      (void)hipFree(set.packed);
       return fail((int)e, "copying encode tables failed: %s", hipGetErrorString(e));
     }
-    set.inv1 = off.inv1;
-    set.inv2 = off.inv2;
-    set.fwd1 = off.fwd1;
-    set.fwd2 = off.fwd2;
-    set.twist = off.twist;
-    set.roots = off.roots;
+    set.tabs = new EncodeTabs(make_encode_tabs(set.base, off, set.packed, pk));
     it = st->encode.emplace(key, set).first;
   }
   *out = &it->second;
@@ -71,20 +119,34 @@ static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32
                        uint32_t* d_out, int layout, hipStream_t stream) {
   const EncodeTableSet* set = nullptr;
   ZODA_TRY(get_encode_tables(st, PL::L, PL::R1, PL::R2, PL::RD, &set));
-  EncodeTabs T{set->base + set->inv1, set->base + set->inv2, set->base + set->fwd1,
-               set->base + set->fwd2, set->base + set->twist, set->base + set->roots};
+  const EncodeTabs& T = *set->tabs;
+  if (PL::TWL && T.packed_words > (uint32_t)PL::TW_WORDS)
+    return fail(ZODA_ESTATE, "encode tables (%u words) exceed the plan's LDS share", T.packed_words);
   const size_t lds_bytes = (size_t)PL::LDS_WORDS * sizeof(int32_t);
   const bool full = cols % PL::W == 0 && k == (uint32_t)PL::S;
-  const void* fn = full ? (const void*)encode_kernel<PL, true> : (const void*)encode_kernel<PL, false>;
+  const bool row = layout == ZODA_LAYOUT_ROW;
+  if constexpr (PL::PERSIST) {
+    if (full && row) {
+      const void* pfn = (const void*)encode_persistent<PL>;
+      ZODA_HIP(hipFuncSetAttribute(pfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+      int per_cu = 0, cus = 0;
+      ZODA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_persistent<PL>, PL::NT, lds_bytes));
+      ZODA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, st->device));
+      uint32_t ntiles = cols / PL::W;
+      uint32_t grid = (uint32_t)std::max(1, per_cu * cus);
+      if (grid > ntiles) grid = ntiles;
+      void* pargs[] = {(void*)&d_in, (void*)&d_out, (void*)&cols, (void*)&ntiles, (void*)&T};
+      ZODA_HIP(hipLaunchKernel(pfn, dim3(grid), dim3(PL::NT), pargs, lds_bytes, stream));
+      return 0;
+    }
+  }
+  const void* fn = full ? (row ? (const void*)encode_kernel<PL, true, 0> : (const void*)encode_kernel<PL, true, 1>)
+                        : (row ? (const void*)encode_kernel<PL, false, 0> : (const void*)encode_kernel<PL, false, 1>);
   if (lds_bytes > 64 * 1024)
     ZODA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
   const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
-  if (full)
-    hipLaunchKernelGGL((encode_kernel<PL, true>), dim3(ntiles), dim3(PL::NT), lds_bytes, stream, d_in,
-                       d_out, k, cols, layout, T);
-  else
-    hipLaunchKernelGGL((encode_kernel<PL, false>), dim3(ntiles), dim3(PL::NT), lds_bytes, stream, d_in,
-                       d_out, k, cols, layout, T);
+  void* args[] = {(void*)&d_in, (void*)&d_out, (void*)&k, (void*)&cols, (void*)&T};
+  ZODA_HIP(hipLaunchKernel(fn, dim3(ntiles), dim3(PL::NT), args, lds_bytes, stream));
   ZODA_HIP(hipGetLastError());
   return 0;
 }

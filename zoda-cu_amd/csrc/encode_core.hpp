@@ -44,19 +44,47 @@
 
 namespace zoda {
 
-// Device (or host, for the emulation) pointers into one table buffer.
+// Device (or host, for the emulation) pointers into one table buffer.  Each
+// twiddle table exists twice: as (w, w/p) double pairs, read from global
+// memory, and as canonical u32 in one packed block that plans with TWL copy
+// into LDS (w/p is then recomputed with one multiply).
 struct EncodeTabs {
-  const Tw* inv1;   // INTT twiddles of the top digit, [low << R1 | k]
-  const Tw* inv2;   // INTT twiddles of the second digit, [low << R2 | k]
+  // INTT twiddles, [high << (LO+R) | low << R | k] for the digit's row bits
+  // `low` below it and `high` above it.  The last INTT pass before the middle
+  // one also carries the part of the twist that is constant over a middle
+  // group (see build_encode_tables), so its table has one entry per row.
+  const Tw* inv1;
+  const Tw* inv2;
   const Tw* fwd1;   // NTT twiddles of the top digit, [k << LO1 | low]
   const Tw* fwd2;   // NTT twiddles of the second digit, [k << LO2 | low]
-  const Tw* twist;  // s entries in storage order: s^-1 * w_2s^(digitrev(i))
-  const Tw* roots;  // 64 entries: w_32^e for e < 32, then w_32^-e
+  const Tw* twist;  // one-pass plans only: s entries, s^-1 * w_2s^i
+  const Tw* roots;  // 128 entries: w_64^e for e < 64, then w_64^-e
+  const uint32_t* packed;  // u32 copies: inv1 | inv2 | fwd1 | fwd2 | twist
+  uint32_t o_inv1, o_inv2, o_fwd1, o_fwd2, o_twist, packed_words;
 };
 
-template <int L_, int W_, int NT_, int R1_, int R2_, int RD_>
+// One twiddle table as a pass reads it.
+struct TwTab {
+  const Tw* g;         // global (w, w/p)
+  const uint32_t* s;   // LDS u32 (plans with TWL)
+};
+
+template <bool LDS_TW>
+ZHD Tw tw_get(const TwTab& t, uint32_t i) {
+  if constexpr (LDS_TW) {
+    const double w = (double)t.s[i];
+    return Tw{w, w * PINV};
+  } else {
+    return t.g[i];
+  }
+}
+
+template <int L_, int W_, int NT_, int R1_, int R2_, int RD_, bool TWL_ = false,
+          bool PERSIST_ = false>
 struct EncodePlan {
   static constexpr int L = L_, W = W_, NT = NT_, R1 = R1_, R2 = R2_, RD = RD_;
+  static constexpr bool TWL = TWL_;          // twiddle tables in LDS
+  static constexpr bool PERSIST = PERSIST_;  // persistent kernel with LDS prefetch
   static_assert(R1 + R2 + RD == L, "digits must cover the row index");
   static_assert(R2 == 0 || R1 > 0, "digits fill from the top");
   static_assert(RD <= 5 && R1 <= 5 && R2 <= 5, "register DFTs are at most radix 32");
@@ -64,7 +92,10 @@ struct EncodePlan {
   static constexpr int D = 1 + (R1 > 0) + (R2 > 0);
   static constexpr int LO2 = RD;
   static constexpr int LO1 = RD + R2;
-  static constexpr int LDS_WORDS = D > 1 ? W * S : 0;
+  static constexpr int TILE_WORDS = D > 1 ? W * S : 0;
+  // u32 words of the packed tables (see build_encode_tables)
+  static constexpr int TW_WORDS = D == 1 ? S : D == 2 ? 2 * S : 3 * S + (1 << LO1);
+  static constexpr int LDS_WORDS = TILE_WORDS + (TWL ? TW_WORDS : 0);
   static constexpr int STEPS = 2 * D - 1;
 };
 
@@ -75,7 +106,7 @@ ZHD constexpr int brev_bits(int x, int bits) {
 }
 
 // Natural-order DFT of 2^R integer-valued doubles: X[k] = sum_m x[m] w^(m k)
-// with w = rt[32 >> R] (a primitive 2^R-th root; rt[e] = w_32^(+-e)).
+// with w = rt[64 >> R] (a primitive 2^R-th root; rt[e] = w_64^(+-e)).
 // Radix-2 decimation in frequency; the bit-reversed result is renamed back to
 // natural order at no cost.  |x| < B on entry gives |X| < 2^R * B.
 template <int R>
@@ -94,7 +125,7 @@ ZHD void dft_reg(double (&x)[1 << R], const Tw* __restrict__ rt) {
         if (j == 0) {
           x[b + j + half] = d;
         } else {
-          const Tw t = rt[j << (4 - hs)];  // w_(2 half)^j = w_32^(j * 32 / (2 half))
+          const Tw t = rt[j << (5 - hs)];  // w_(2 half)^j = w_64^(j * 64 / (2 half))
           x[b + j + half] = mulmod(d, t.w, t.wq);
         }
       }
@@ -119,42 +150,63 @@ ZHD size_t out_index(uint32_t row, uint32_t cidx, uint32_t cols, uint32_t out_ro
   return layout == 0 ? (size_t)row * cols + cidx : (size_t)cidx * out_rows + row;
 }
 
+struct NoRelease {
+  ZHD void operator()() const {}
+};
+
 // One pass for thread `tid` of the work-group encoding tile `tile`.
 //   LO, R    the digit this pass transforms
 //   KIND     PASS_INTT / PASS_MID / PASS_NTT
-//   FIRST    read the tile from global memory (and write the even rows)
+//   FIRST    read the tile (from global memory, or with PERSIST from the
+//            copy prefetched into LDS) and write the even output rows
 //   LAST     write canonical odd rows to global memory
 //   LOA, RA  digit above, whose forward twiddle ends MID/NTT passes (RA = 0: none)
-template <class PL, int LO, int R, int KIND, bool FIRST, bool LAST, int LOA, int RA, bool FULL>
+//   FULL     whole tile and k = 2^L: no bounds checks
+//   PERSIST  (LAST) read every group of the thread from LDS first and call
+//            release() -- the caller reuses the LDS tile for the next prefetch
+template <class PL, int LO, int R, int KIND, bool FIRST, bool LAST, int LOA, int RA, bool FULL,
+          bool PERSIST, class Release>
 ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
                      uint32_t* __restrict__ out, uint32_t krows, uint32_t cols, int layout,
-                     int32_t* __restrict__ lds, const EncodeTabs& T, const Tw* __restrict__ own,
-                     const Tw* __restrict__ above) {
+                     int32_t* __restrict__ lds, const EncodeTabs& T, const TwTab own,
+                     const TwTab above, const TwTab twist, const Release& release) {
   constexpr int RR = 1 << R;
   constexpr int NGROUPS = PL::W << (PL::L - R);
   static_assert(NGROUPS % PL::NT == 0, "groups must divide evenly over the threads");
   constexpr int G = NGROUPS / PL::NT;
   constexpr uint32_t OUT_ROWS = 2u * PL::S;
-  const Tw* rinv = T.roots + 32;
+  const Tw* rinv = T.roots + 64;
   const Tw* rfwd = T.roots;
+  const uint32_t* lds_u = reinterpret_cast<const uint32_t*>(lds);
 
   // The first pass loads up to 32 values per thread before computing, in
   // chunks of PB groups, to keep loads in flight without a large register set.
-  constexpr int PB = FIRST ? (G * RR <= 32 ? G : (RR >= 32 ? 1 : 32 / RR)) : G;
+  // A persistent last pass holds all of its values before release().
+  constexpr bool HOLD = LAST && PERSIST;
+  constexpr int PB = HOLD ? G : FIRST ? (G * RR <= 32 ? G : (RR >= 32 ? 1 : 32 / RR)) : G;
   static_assert(G % PB == 0, "preload chunks must divide the groups");
 #pragma unroll
   for (int gc = 0; gc < G; gc += PB) {
-    uint32_t raw[FIRST ? PB : 1][FIRST ? RR : 1];
+    uint32_t raw[(FIRST || HOLD) ? PB : 1][(FIRST || HOLD) ? RR : 1];
     if constexpr (FIRST) {
+      // Rows of one group are 2^LO apart: one base address per group and a
+      // uniform stride per value.
+      const size_t istep = (size_t)cols << LO;
+      const size_t ostep = layout == 0 ? (size_t)cols << (LO + 1) : (size_t)2 << LO;
 #pragma unroll
       for (int gi = 0; gi < PB; gi++) {
         const uint32_t g = (uint32_t)((gc + gi) * PL::NT + tid);
         const uint32_t c = g % PL::W, u = g / PL::W;
         const uint32_t cidx = tile * PL::W + c;
+        const uint32_t j0 = row_of<LO, R>(u, 0);
+        const uint32_t* src = in + (size_t)j0 * cols + cidx;
 #pragma unroll
         for (int m = 0; m < RR; m++) {
-          const uint32_t j = row_of<LO, R>(u, m);
-          raw[gi][m] = (FULL || (cidx < cols && j < krows)) ? in[(size_t)j * cols + cidx] : 0u;
+          const uint32_t j = j0 + ((uint32_t)m << LO);
+          if constexpr (PERSIST)
+            raw[gi][m] = lds_u[j * PL::W + c];
+          else
+            raw[gi][m] = (FULL || (cidx < cols && j < krows)) ? src[(size_t)m * istep] : 0u;
         }
       }
 #pragma unroll
@@ -163,104 +215,179 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
         const uint32_t c = g % PL::W, u = g / PL::W;
         const uint32_t cidx = tile * PL::W + c;
         if (FULL || cidx < cols) {
+          uint32_t* dst = out + out_index(2 * row_of<LO, R>(u, 0), cidx, cols, OUT_ROWS, layout);
+#pragma unroll
+          for (int m = 0; m < RR; m++) dst[(size_t)m * ostep] = raw[gi][m];
+        }
+      }
+    }
+    if constexpr (HOLD) {
+#pragma unroll
+      for (int gi = 0; gi < PB; gi++) {
+        const uint32_t g = (uint32_t)((gc + gi) * PL::NT + tid);
+        const uint32_t c = g % PL::W, u = g / PL::W;
+#pragma unroll
+        for (int m = 0; m < RR; m++) raw[gi][m] = lds_u[row_of<LO, R>(u, m) * PL::W + c];
+      }
+      release();
+    }
+#pragma unroll
+    for (int gi = 0; gi < PB; gi++) {
+      const uint32_t g = (uint32_t)((gc + gi) * PL::NT + tid);
+      const uint32_t c = g % PL::W, u = g / PL::W;
+      double x[RR];
+#pragma unroll
+      for (int m = 0; m < RR; m++) {
+        if constexpr (FIRST)
+          x[m] = (double)raw[gi][m];
+        else if constexpr (HOLD)
+          x[m] = (double)(int32_t)raw[gi][m];
+        else
+          x[m] = (double)lds[row_of<LO, R>(u, m) * PL::W + c];
+      }
+      if constexpr (KIND == PASS_INTT) {
+        dft_reg<R>(x, rinv);
+        const uint32_t base = ((u >> LO) << (LO + R)) | ((u & ((1u << LO) - 1u)) << R);
+        constexpr bool LAST_INTT = PL::D == 2 || (PL::D == 3 && LO == PL::LO2);
+        if constexpr (LAST_INTT) {
+          // the table folds in the twist's per-group factor
+#pragma unroll
+          for (int k = 0; k < RR; k++) x[k] = mulmod(x[k], tw_get<PL::TWL>(own, base + k));
+        } else {
+          x[0] = reduce(x[0]);  // twiddle w^0 = 1
+#pragma unroll
+          for (int k = 1; k < RR; k++) x[k] = mulmod(x[k], tw_get<PL::TWL>(own, base + k));
+        }
+      } else {
+        if constexpr (KIND == PASS_MID) {
+          dft_reg<R>(x, rinv);
+          if constexpr (PL::D == 1) {
+#pragma unroll
+            for (int m = 0; m < RR; m++)
+              x[m] = mulmod(x[m], tw_get<PL::TWL>(twist, row_of<LO, R>(u, m)));
+          } else {
+            // rest of the twist: w_(2^(RD+1))^m, a constant per value
+#pragma unroll
+            for (int m = 1; m < RR; m++) {
+              const Tw t = rfwd[m << (5 - R)];
+              x[m] = mulmod(x[m], t.w, t.wq);
+            }
+          }
+        }
+        dft_reg<R>(x, rfwd);
+        if constexpr (!LAST) {
+          static_assert(RA > 0, "a non-final pass applies the twiddle of the digit above");
 #pragma unroll
           for (int m = 0; m < RR; m++) {
             const uint32_t j = row_of<LO, R>(u, m);
-            out[out_index(2 * j, cidx, cols, OUT_ROWS, layout)] = raw[gi][m];
+            const uint32_t k = (j >> LOA) & ((1u << RA) - 1u);
+            x[m] = mulmod(x[m], tw_get<PL::TWL>(above, (k << LOA) | (j & ((1u << LOA) - 1u))));
           }
         }
       }
-    }
+      if constexpr (LAST) {
+        const uint32_t cidx = tile * PL::W + c;
+        if (FULL || cidx < cols) {
+          const size_t ostep = layout == 0 ? (size_t)cols << (LO + 1) : (size_t)2 << LO;
+          uint32_t* dst = out + out_index(2 * row_of<LO, R>(u, 0) + 1, cidx, cols, OUT_ROWS, layout);
 #pragma unroll
-  for (int gi = 0; gi < PB; gi++) {
-    const uint32_t g = (uint32_t)((gc + gi) * PL::NT + tid);
-    const uint32_t c = g % PL::W, u = g / PL::W;
-    double x[RR];
-#pragma unroll
-    for (int m = 0; m < RR; m++) {
-      if constexpr (FIRST) {
-        x[m] = (double)raw[gi][m];
+          for (int m = 0; m < RR; m++) dst[(size_t)m * ostep] = canonical(x[m]);
+        }
       } else {
-        x[m] = (double)lds[row_of<LO, R>(u, m) * PL::W + c];
+#pragma unroll
+        for (int m = 0; m < RR; m++) lds[row_of<LO, R>(u, m) * PL::W + c] = (int32_t)x[m];
       }
-    }
-    if constexpr (KIND == PASS_INTT) {
-      dft_reg<R>(x, rinv);
-      const Tw* tw = own + ((u & ((1u << LO) - 1u)) << R);
-      x[0] = reduce(x[0]);  // twiddle w^0 = 1
-#pragma unroll
-      for (int k = 1; k < RR; k++) x[k] = mulmod(x[k], tw[k]);
-    } else {
-      if constexpr (KIND == PASS_MID) {
-        dft_reg<R>(x, rinv);
-#pragma unroll
-        for (int m = 0; m < RR; m++) x[m] = mulmod(x[m], T.twist[row_of<LO, R>(u, m)]);
-      }
-      dft_reg<R>(x, rfwd);
-      if constexpr (!LAST) {
-        static_assert(RA > 0, "a non-final pass applies the twiddle of the digit above");
-#pragma unroll
-        for (int m = 0; m < RR; m++) {
-          const uint32_t j = row_of<LO, R>(u, m);
-          const uint32_t k = (j >> LOA) & ((1u << RA) - 1u);
-          x[m] = mulmod(x[m], above[(k << LOA) | (j & ((1u << LOA) - 1u))]);
-        }
-      }
-    }
-    if constexpr (LAST) {
-      const uint32_t cidx = tile * PL::W + c;
-      if (FULL || cidx < cols) {
-#pragma unroll
-        for (int m = 0; m < RR; m++) {
-          const uint32_t j = row_of<LO, R>(u, m);
-          out[out_index(2 * j + 1, cidx, cols, OUT_ROWS, layout)] = canonical(x[m]);
-        }
-      }
-    } else {
-#pragma unroll
-      for (int m = 0; m < RR; m++) lds[row_of<LO, R>(u, m) * PL::W + c] = (int32_t)x[m];
     }
   }
+}
+
+// Copies the packed u32 twiddle tables behind the tile in LDS (plans with
+// TWL); the caller puts a work-group barrier between this and step 0.
+template <class PL>
+ZHD void encode_load_tables(int tid, int32_t* __restrict__ lds, const EncodeTabs& T) {
+  if constexpr (PL::TWL) {
+    uint32_t* dst = reinterpret_cast<uint32_t*>(lds + PL::TILE_WORDS);
+    for (uint32_t i = (uint32_t)tid; i < T.packed_words; i += PL::NT) dst[i] = T.packed[i];
   }
 }
 
 // Step STEP (0 .. 2D-2) of the encode of one tile; steps are separated by a
 // work-group barrier on the GPU.  FULL: every column of the tile exists and
 // k = 2^L, so no bounds checks are emitted.
-template <class PL, int STEP, bool FULL>
+template <class PL, int STEP, bool FULL, bool PERSIST = false, class Release = NoRelease>
 ZHD void encode_step(int tid, uint32_t tile, const uint32_t* __restrict__ in,
                      uint32_t* __restrict__ out, uint32_t krows, uint32_t cols, int layout,
-                     int32_t* __restrict__ lds, const EncodeTabs& T) {
+                     int32_t* __restrict__ lds, const EncodeTabs& T,
+                     const Release& release = Release()) {
   constexpr int L1 = PL::LO1, L2 = PL::LO2;
+  const uint32_t* sb = PL::TWL ? reinterpret_cast<const uint32_t*>(lds + PL::TILE_WORDS) : nullptr;
+  const TwTab none{nullptr, nullptr};
+  const TwTab inv1{T.inv1, PL::TWL ? sb + T.o_inv1 : nullptr};
+  const TwTab inv2{T.inv2, PL::TWL ? sb + T.o_inv2 : nullptr};
+  const TwTab fwd1{T.fwd1, PL::TWL ? sb + T.o_fwd1 : nullptr};
+  const TwTab fwd2{T.fwd2, PL::TWL ? sb + T.o_fwd2 : nullptr};
+  const TwTab twist{T.twist, PL::TWL ? sb + T.o_twist : nullptr};
+  static_assert(!PERSIST || PL::D > 1, "persistent plans keep the tile in LDS");
   if constexpr (PL::D == 1) {
-    encode_pass<PL, 0, PL::RD, PASS_MID, true, true, 0, 0, FULL>(tid, tile, in, out, krows, cols, layout,
-                                                           lds, T, nullptr, nullptr);
+    encode_pass<PL, 0, PL::RD, PASS_MID, true, true, 0, 0, FULL, false>(
+        tid, tile, in, out, krows, cols, layout, lds, T, none, none, twist, release);
   } else if constexpr (PL::D == 2) {
     if constexpr (STEP == 0)
-      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL>(tid, tile, in, out, krows, cols,
-                                                                layout, lds, T, T.inv1, nullptr);
+      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL, PERSIST>(
+          tid, tile, in, out, krows, cols, layout, lds, T, inv1, none, none, release);
     else if constexpr (STEP == 1)
-      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L1, PL::R1, FULL>(
-          tid, tile, in, out, krows, cols, layout, lds, T, nullptr, T.fwd1);
+      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L1, PL::R1, FULL, PERSIST>(
+          tid, tile, in, out, krows, cols, layout, lds, T, none, fwd1, twist, release);
     else
-      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL>(tid, tile, in, out, krows, cols,
-                                                               layout, lds, T, nullptr, nullptr);
+      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL, PERSIST>(
+          tid, tile, in, out, krows, cols, layout, lds, T, none, none, none, release);
   } else {
     if constexpr (STEP == 0)
-      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL>(tid, tile, in, out, krows, cols,
-                                                                layout, lds, T, T.inv1, nullptr);
+      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL, PERSIST>(
+          tid, tile, in, out, krows, cols, layout, lds, T, inv1, none, none, release);
     else if constexpr (STEP == 1)
-      encode_pass<PL, L2, PL::R2, PASS_INTT, false, false, 0, 0, FULL>(tid, tile, in, out, krows, cols,
-                                                                 layout, lds, T, T.inv2, nullptr);
+      encode_pass<PL, L2, PL::R2, PASS_INTT, false, false, 0, 0, FULL, PERSIST>(
+          tid, tile, in, out, krows, cols, layout, lds, T, inv2, none, none, release);
     else if constexpr (STEP == 2)
-      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L2, PL::R2, FULL>(
-          tid, tile, in, out, krows, cols, layout, lds, T, nullptr, T.fwd2);
+      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L2, PL::R2, FULL, PERSIST>(
+          tid, tile, in, out, krows, cols, layout, lds, T, none, fwd2, twist, release);
     else if constexpr (STEP == 3)
-      encode_pass<PL, L2, PL::R2, PASS_NTT, false, false, L1, PL::R1, FULL>(
-          tid, tile, in, out, krows, cols, layout, lds, T, nullptr, T.fwd1);
+      encode_pass<PL, L2, PL::R2, PASS_NTT, false, false, L1, PL::R1, FULL, PERSIST>(
+          tid, tile, in, out, krows, cols, layout, lds, T, none, fwd1, none, release);
     else
-      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL>(tid, tile, in, out, krows, cols,
-                                                               layout, lds, T, nullptr, nullptr);
+      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL, PERSIST>(
+          tid, tile, in, out, krows, cols, layout, lds, T, none, none, none, release);
+  }
+}
+
+// Copies tile `tile` of the row-major input into the LDS tile, in the
+// [row][column] layout the passes read (u32, FULL tiles only).  On the GPU
+// each lane moves 16 bytes straight from global memory into LDS
+// (global_load_lds_dwordx4): a wave instruction fills 64 lanes * 16 bytes =
+// 256/W consecutive rows, so the LDS destination is contiguous per wave as
+// the instruction requires.  Completion is tracked by vmcnt.
+template <class PL>
+ZHD void prefetch_tile(int tid, uint32_t tile, const uint32_t* __restrict__ in, uint32_t cols,
+                       int32_t* __restrict__ lds) {
+  constexpr int LPR = PL::W / 4;   // lanes per row
+  constexpr int RPI = 64 / LPR;    // rows per wave instruction
+  constexpr int NWAVE = PL::NT / 64;
+  constexpr int NI = PL::S / RPI;  // wave instructions per tile
+  static_assert(PL::W % 4 == 0 && 64 % LPR == 0 && PL::S % RPI == 0 && NI % NWAVE == 0,
+                "tile shape does not split into whole wave loads");
+  const int wave = tid / 64, lane = tid % 64;
+  const uint32_t c4 = (uint32_t)(lane % LPR) * 4;
+#pragma unroll
+  for (int i = 0; i < NI / NWAVE; i++) {
+    const uint32_t row0 = (uint32_t)((wave * (NI / NWAVE) + i) * RPI);
+    const uint32_t j = row0 + (uint32_t)(lane / LPR);
+    const uint32_t* src = in + (size_t)j * cols + (size_t)tile * PL::W + c4;
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds(
+        (const void*)src, (__attribute__((address_space(3))) void*)(lds + row0 * PL::W), 16, 0, 0);
+#else
+    for (int q = 0; q < 4; q++) lds[j * PL::W + c4 + q] = (int32_t)src[q];
+#endif
   }
 }
 

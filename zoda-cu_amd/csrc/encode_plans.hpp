@@ -1,6 +1,8 @@
 // Tile width, work-group size and digit split of the fused encode for each
 // side s = 2^L.  Tiles are LDS-resident as int32 (W * s * 4 bytes <= 128 KiB);
-// sides up to 32 keep a whole column in one thread's registers (no LDS).
+// sides up to 32 keep a whole column in one thread's registers (no LDS).  The
+// flags keep the twiddle tables in LDS as u32 next to the tile, and run whole
+// tiles in a persistent kernel that prefetches each next tile into LDS.
 #pragma once
 
 #include "encode_core.hpp"
@@ -14,12 +16,12 @@ using Plan2 = EncodePlan<2, 256, 256, 0, 0, 2>;
 using Plan3 = EncodePlan<3, 256, 256, 0, 0, 3>;
 using Plan4 = EncodePlan<4, 256, 256, 0, 0, 4>;
 using Plan5 = EncodePlan<5, 256, 256, 0, 0, 5>;
-using Plan6 = EncodePlan<6, 64, 256, 3, 0, 3>;
-using Plan7 = EncodePlan<7, 64, 256, 4, 0, 3>;
-using Plan8 = EncodePlan<8, 64, 512, 4, 0, 4>;
-using Plan9 = EncodePlan<9, 32, 512, 3, 3, 3>;
-using Plan10 = EncodePlan<10, 32, 1024, 4, 3, 3>;
-using Plan11 = EncodePlan<11, 16, 1024, 4, 4, 3>;
+using Plan6 = EncodePlan<6, 64, 256, 3, 0, 3, true>;
+using Plan7 = EncodePlan<7, 64, 256, 4, 0, 3, true>;
+using Plan8 = EncodePlan<8, 64, 512, 4, 0, 4, true>;
+using Plan9 = EncodePlan<9, 32, 512, 3, 3, 3, true>;
+using Plan10 = EncodePlan<10, 32, 1024, 4, 3, 3, true, true>;
+using Plan11 = EncodePlan<11, 16, 1024, 4, 4, 3, true, true>;
 using Plan12 = EncodePlan<12, 8, 1024, 4, 4, 4>;
 
 constexpr int ENCODE_MAX_LOG = 12;
