@@ -46,19 +46,20 @@ static int sim_plan(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out
 }
 
 template <class PL, int STEP>
-static void run_persistent_steps(uint32_t tile, const uint32_t* in, uint32_t* out, uint32_t cols,
-                                 int32_t* lds, const EncodeTabs& T) {
+static void run_late_steps(uint32_t tile, const uint32_t* in, uint32_t* out, uint32_t cols,
+                           int32_t* lds, const EncodeTabs& T) {
   for (int tid = 0; tid < PL::NT; tid++)
-    encode_step<PL, STEP, true, true>(tid, tile, in, out, (uint32_t)PL::S, cols, 0, lds, T);
-  if constexpr (STEP + 1 < PL::STEPS) run_persistent_steps<PL, STEP + 1>(tile, in, out, cols, lds, T);
+    encode_step<PL, STEP, true>(tid, tile, in, out, (uint32_t)PL::S, cols, 0, lds, T);
+  if constexpr (STEP + 1 < PL::STEPS) run_late_steps<PL, STEP + 1>(tile, in, out, cols, lds, T);
 }
 
-// The persistent kernel's order of work on one work-group: tables, prefetch of
-// the first tile, then per tile all passes (the last pass holds its values
-// before the next prefetch overwrites the tile, which is what running every
-// thread's pass to completion first models) and the next prefetch.
+// The pipelined persistent kernel's order of work on one work-group: the
+// first pass from the registers loaded ahead, the second pass with the
+// even-row stores of those registers, the loads of the next tile into the
+// same registers, the remaining passes.
 template <class PL>
-static int sim_persistent(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out) {
+static int sim_pipelined(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out) {
+  using PP = Pipe<PL>;
   if (cols % PL::W != 0 || k != (uint32_t)PL::S) return -3;
   EncodeTabOffsets off;
   std::vector<Tw> tabs = build_encode_tables(PL::L, PL::R1, PL::R2, PL::RD, &off);
@@ -67,23 +68,33 @@ static int sim_persistent(const uint32_t* in, uint32_t k, uint32_t cols, uint32_
   if (pk.words.size() > (size_t)PL::TW_WORDS) return -2;
   std::vector<int32_t> lds(PL::LDS_WORDS);
   for (int tid = 0; tid < PL::NT; tid++) encode_load_tables<PL>(tid, lds.data(), T);
+  const uint32_t* sb = reinterpret_cast<const uint32_t*>(lds.data() + PL::TILE_WORDS);
+  const TwTab inv1{T.inv1, sb + T.o_inv1};
+  struct Regs { uint32_t v[PP::G][PP::RR]; };
+  std::vector<Regs> raw(PL::NT);
   const uint32_t ntiles = cols / PL::W;
-  for (int tid = 0; tid < PL::NT; tid++) prefetch_tile<PL>(tid, 0, in, cols, lds.data());
+  for (int tid = 0; tid < PL::NT; tid++) PP::load(tid, 0, in, cols, raw[tid].v);
   for (uint32_t tile = 0; tile < ntiles; tile++) {
-    run_persistent_steps<PL, 0>(tile, in, out, cols, lds.data(), T);
+    for (int tid = 0; tid < PL::NT; tid++) PP::first(tid, raw[tid].v, lds.data(), T, inv1);
+    for (int tid = 0; tid < PL::NT; tid++)
+      encode_step<PL, 1, true>(tid, tile, in, out, k, cols, 0, lds.data(), T, [&](int gi) {
+        if (gi < PP::G) PP::store_even(tid, gi, tile, out, cols, raw[tid].v[gi]);
+      });
     if (tile + 1 < ntiles)
-      for (int tid = 0; tid < PL::NT; tid++) prefetch_tile<PL>(tid, tile + 1, in, cols, lds.data());
+      for (int tid = 0; tid < PL::NT; tid++) PP::load(tid, tile + 1, in, cols, raw[tid].v);
+    run_late_steps<PL, 2>(tile, in, out, cols, lds.data(), T);
   }
   return 0;
 }
 
-// Persistent form of the single-square plan of side k (whole tiles only).
+// Persistent (pipelined) form of the single-square plan of side k (whole
+// tiles only).
 extern "C" int zoda_sim_encode_persistent(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out) {
   uint32_t nk = 1, L = 0;
   while (nk < k) { nk <<= 1; L++; }
   return with_encode_plan((int)L, [&](auto plan) -> int {
     using PL = decltype(plan);
-    if constexpr (PL::PERSIST) return sim_persistent<PL>(in, k, cols, out);
+    if constexpr (PL::PERSIST) return sim_pipelined<PL>(in, k, cols, out);
     else return -4;
   });
 }

@@ -47,40 +47,46 @@ __global__ __launch_bounds__(PL::NT) void encode_kernel(const uint32_t* __restri
   }
 }
 
-// Persistent form for whole tiles: one work-group per CU walks its tiles
-// (XCD-contiguous, as above), keeps the twiddle tables in LDS for all of them,
-// and prefetches the next tile's input straight into LDS while the last pass
-// of the current tile computes and stores.
+// Persistent form with the first pass pipelined through registers (Pipe):
+// the loads of the next tile are issued once the current tile's values have
+// left the registers (after its second pass) and complete behind the
+// remaining passes; the even-row stores are spread over the second pass.
 template <class PL>
-__global__ __launch_bounds__(PL::NT) void encode_persistent(const uint32_t* __restrict__ in,
-                                                            uint32_t* __restrict__ out,
-                                                            uint32_t cols, uint32_t ntiles,
-                                                            EncodeTabs T) {
+__global__ __launch_bounds__(PL::NT) void encode_pipelined(const uint32_t* __restrict__ in,
+                                                           uint32_t* __restrict__ out,
+                                                           uint32_t cols, uint32_t ntiles,
+                                                           EncodeTabs T) {
+  using PP = Pipe<PL>;
+  constexpr int G1 = (PL::W << (PL::L - (PL::D == 2 ? PL::RD : PL::R2))) / PL::NT;
+  static_assert(G1 >= PP::G, "the second pass must have a group for every first-pass group");
   extern __shared__ int32_t lds[];
-  const int tid = (int)threadIdx.x;
-  encode_load_tables<PL>(tid, lds, T);
+  encode_load_tables<PL>((int)threadIdx.x, lds, T);
+  const uint32_t* sb = reinterpret_cast<const uint32_t*>(lds + PL::TILE_WORDS);
+  const TwTab inv1{T.inv1, sb + T.o_inv1};
+  const uint32_t k = (uint32_t)PL::S;
+  uint32_t raw[PP::G][PP::RR];
   uint32_t l = blockIdx.x;
-  if (l < ntiles) prefetch_tile<PL>(tid, xcd_tile(l, ntiles), in, cols, lds);
+  if (l < ntiles) PP::load((int)threadIdx.x, xcd_tile(l, ntiles), in, cols, raw);
+  __syncthreads();
   for (; l < ntiles; l += gridDim.x) {
     const uint32_t tile = xcd_tile(l, ntiles);
     const uint32_t next = l + gridDim.x;
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's prefetch (and tables) landed
-    __syncthreads();                // ... and every other wave's
-    const uint32_t k = (uint32_t)PL::S;
-    encode_step<PL, 0, true, true>(tid, tile, in, out, k, cols, 0, lds, T);
+    const int tid = (int)threadIdx.x;
+    PP::first(tid, raw, lds, T, inv1);
     __syncthreads();
-    encode_step<PL, 1, true, true>(tid, tile, in, out, k, cols, 0, lds, T);
+    encode_step<PL, 1, true>(tid, tile, in, out, k, cols, 0, lds, T, [&](int gi) {
+      if (gi < PP::G) PP::store_even(tid, gi, tile, out, cols, raw[gi]);
+    });
+    if (next < ntiles) PP::load(tid, xcd_tile(next, ntiles), in, cols, raw);
     __syncthreads();
-    encode_step<PL, 2, true, true>(tid, tile, in, out, k, cols, 0, lds, T);
+    encode_step<PL, 2, true>(tid, tile, in, out, k, cols, 0, lds, T);
     if constexpr (PL::STEPS > 3) {
       __syncthreads();
-      encode_step<PL, 3, true, true>(tid, tile, in, out, k, cols, 0, lds, T);
+      encode_step<PL, 3, true>(tid, tile, in, out, k, cols, 0, lds, T);
       __syncthreads();
-      encode_step<PL, 4, true, true>(tid, tile, in, out, k, cols, 0, lds, T, [&] {
-        __syncthreads();  // every wave holds its last-pass values: the tile is free
-        if (next < ntiles) prefetch_tile<PL>(tid, xcd_tile(next, ntiles), in, cols, lds);
-      });
+      encode_step<PL, 4, true>(tid, tile, in, out, k, cols, 0, lds, T);
     }
+    __syncthreads();  // the next tile's first pass writes LDS
   }
 }
 
@@ -127,10 +133,10 @@ static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32
   const bool row = layout == ZODA_LAYOUT_ROW;
   if constexpr (PL::PERSIST) {
     if (full && row) {
-      const void* pfn = (const void*)encode_persistent<PL>;
+      const void* pfn = (const void*)encode_pipelined<PL>;
       ZODA_HIP(hipFuncSetAttribute(pfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
       int per_cu = 0, cus = 0;
-      ZODA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_persistent<PL>, PL::NT, lds_bytes));
+      ZODA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_pipelined<PL>, PL::NT, lds_bytes));
       ZODA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, st->device));
       uint32_t ntiles = cols / PL::W;
       uint32_t grid = (uint32_t)std::max(1, per_cu * cus);

@@ -80,11 +80,12 @@ ZHD Tw tw_get(const TwTab& t, uint32_t i) {
 }
 
 template <int L_, int W_, int NT_, int R1_, int R2_, int RD_, bool TWL_ = false,
-          bool PERSIST_ = false>
+          bool PERSIST_ = false, bool FENCE_ = false>
 struct EncodePlan {
   static constexpr int L = L_, W = W_, NT = NT_, R1 = R1_, R2 = R2_, RD = RD_;
   static constexpr bool TWL = TWL_;          // twiddle tables in LDS
-  static constexpr bool PERSIST = PERSIST_;  // persistent kernel with LDS prefetch
+  static constexpr bool PERSIST = PERSIST_;  // persistent kernel, first pass pipelined in registers
+  static constexpr bool FENCE = FENCE_;      // one group's values live at a time
   static_assert(R1 + R2 + RD == L, "digits must cover the row index");
   static_assert(R2 == 0 || R1 > 0, "digits fill from the top");
   static_assert(RD <= 5 && R1 <= 5 && R2 <= 5, "register DFTs are at most radix 32");
@@ -140,6 +141,14 @@ ZHD void dft_reg(double (&x)[1 << R], const Tw* __restrict__ rt) {
 
 enum PassKind { PASS_INTT = 0, PASS_MID = 1, PASS_NTT = 2 };
 
+// Keeps the compiler from interleaving the work of consecutive groups, which
+// bounds a pass's live registers to about one group's values.
+ZHD void group_fence() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
 // Row index of digit value m in group u for a digit of R bits at bit LO.
 template <int LO, int R>
 ZHD uint32_t row_of(uint32_t u, uint32_t m) {
@@ -150,26 +159,25 @@ ZHD size_t out_index(uint32_t row, uint32_t cidx, uint32_t cols, uint32_t out_ro
   return layout == 0 ? (size_t)row * cols + cidx : (size_t)cidx * out_rows + row;
 }
 
-struct NoRelease {
-  ZHD void operator()() const {}
+// Called with the group index at the start of every group of a pass.
+struct NoHook {
+  ZHD void operator()(int) const {}
 };
 
 // One pass for thread `tid` of the work-group encoding tile `tile`.
 //   LO, R    the digit this pass transforms
 //   KIND     PASS_INTT / PASS_MID / PASS_NTT
-//   FIRST    read the tile (from global memory, or with PERSIST from the
-//            copy prefetched into LDS) and write the even output rows
+//   FIRST    read the tile from global memory and write the even output rows
 //   LAST     write canonical odd rows to global memory
 //   LOA, RA  digit above, whose forward twiddle ends MID/NTT passes (RA = 0: none)
 //   FULL     whole tile and k = 2^L: no bounds checks
-//   PERSIST  (LAST) read every group of the thread from LDS first and call
-//            release() -- the caller reuses the LDS tile for the next prefetch
+//   hook     called with the group index before each group is transformed
 template <class PL, int LO, int R, int KIND, bool FIRST, bool LAST, int LOA, int RA, bool FULL,
-          bool PERSIST, class Release>
+          class Hook>
 ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
                      uint32_t* __restrict__ out, uint32_t krows, uint32_t cols, int layout,
                      int32_t* __restrict__ lds, const EncodeTabs& T, const TwTab own,
-                     const TwTab above, const TwTab twist, const Release& release) {
+                     const TwTab above, const TwTab twist, const Hook& hook) {
   constexpr int RR = 1 << R;
   constexpr int NGROUPS = PL::W << (PL::L - R);
   static_assert(NGROUPS % PL::NT == 0, "groups must divide evenly over the threads");
@@ -177,17 +185,14 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
   constexpr uint32_t OUT_ROWS = 2u * PL::S;
   const Tw* rinv = T.roots + 64;
   const Tw* rfwd = T.roots;
-  const uint32_t* lds_u = reinterpret_cast<const uint32_t*>(lds);
 
   // The first pass loads up to 32 values per thread before computing, in
   // chunks of PB groups, to keep loads in flight without a large register set.
-  // A persistent last pass holds all of its values before release().
-  constexpr bool HOLD = LAST && PERSIST;
-  constexpr int PB = HOLD ? G : FIRST ? (G * RR <= 32 ? G : (RR >= 32 ? 1 : 32 / RR)) : G;
+  constexpr int PB = FIRST ? (G * RR <= 32 ? G : (RR >= 32 ? 1 : 32 / RR)) : G;
   static_assert(G % PB == 0, "preload chunks must divide the groups");
 #pragma unroll
   for (int gc = 0; gc < G; gc += PB) {
-    uint32_t raw[(FIRST || HOLD) ? PB : 1][(FIRST || HOLD) ? RR : 1];
+    uint32_t raw[FIRST ? PB : 1][FIRST ? RR : 1];
     if constexpr (FIRST) {
       // Rows of one group are 2^LO apart: one base address per group and a
       // uniform stride per value.
@@ -203,10 +208,7 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
 #pragma unroll
         for (int m = 0; m < RR; m++) {
           const uint32_t j = j0 + ((uint32_t)m << LO);
-          if constexpr (PERSIST)
-            raw[gi][m] = lds_u[j * PL::W + c];
-          else
-            raw[gi][m] = (FULL || (cidx < cols && j < krows)) ? src[(size_t)m * istep] : 0u;
+          raw[gi][m] = (FULL || (cidx < cols && j < krows)) ? src[(size_t)m * istep] : 0u;
         }
       }
 #pragma unroll
@@ -221,18 +223,10 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
         }
       }
     }
-    if constexpr (HOLD) {
-#pragma unroll
-      for (int gi = 0; gi < PB; gi++) {
-        const uint32_t g = (uint32_t)((gc + gi) * PL::NT + tid);
-        const uint32_t c = g % PL::W, u = g / PL::W;
-#pragma unroll
-        for (int m = 0; m < RR; m++) raw[gi][m] = lds_u[row_of<LO, R>(u, m) * PL::W + c];
-      }
-      release();
-    }
 #pragma unroll
     for (int gi = 0; gi < PB; gi++) {
+      if constexpr (PL::FENCE) group_fence();
+      hook(gc + gi);
       const uint32_t g = (uint32_t)((gc + gi) * PL::NT + tid);
       const uint32_t c = g % PL::W, u = g / PL::W;
       double x[RR];
@@ -240,8 +234,6 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
       for (int m = 0; m < RR; m++) {
         if constexpr (FIRST)
           x[m] = (double)raw[gi][m];
-        else if constexpr (HOLD)
-          x[m] = (double)(int32_t)raw[gi][m];
         else
           x[m] = (double)lds[row_of<LO, R>(u, m) * PL::W + c];
       }
@@ -314,11 +306,11 @@ ZHD void encode_load_tables(int tid, int32_t* __restrict__ lds, const EncodeTabs
 // Step STEP (0 .. 2D-2) of the encode of one tile; steps are separated by a
 // work-group barrier on the GPU.  FULL: every column of the tile exists and
 // k = 2^L, so no bounds checks are emitted.
-template <class PL, int STEP, bool FULL, bool PERSIST = false, class Release = NoRelease>
+template <class PL, int STEP, bool FULL, class Hook = NoHook>
 ZHD void encode_step(int tid, uint32_t tile, const uint32_t* __restrict__ in,
                      uint32_t* __restrict__ out, uint32_t krows, uint32_t cols, int layout,
                      int32_t* __restrict__ lds, const EncodeTabs& T,
-                     const Release& release = Release()) {
+                     const Hook& hook = Hook()) {
   constexpr int L1 = PL::LO1, L2 = PL::LO2;
   const uint32_t* sb = PL::TWL ? reinterpret_cast<const uint32_t*>(lds + PL::TILE_WORDS) : nullptr;
   const TwTab none{nullptr, nullptr};
@@ -327,68 +319,100 @@ ZHD void encode_step(int tid, uint32_t tile, const uint32_t* __restrict__ in,
   const TwTab fwd1{T.fwd1, PL::TWL ? sb + T.o_fwd1 : nullptr};
   const TwTab fwd2{T.fwd2, PL::TWL ? sb + T.o_fwd2 : nullptr};
   const TwTab twist{T.twist, PL::TWL ? sb + T.o_twist : nullptr};
-  static_assert(!PERSIST || PL::D > 1, "persistent plans keep the tile in LDS");
   if constexpr (PL::D == 1) {
-    encode_pass<PL, 0, PL::RD, PASS_MID, true, true, 0, 0, FULL, false>(
-        tid, tile, in, out, krows, cols, layout, lds, T, none, none, twist, release);
+    encode_pass<PL, 0, PL::RD, PASS_MID, true, true, 0, 0, FULL>(
+        tid, tile, in, out, krows, cols, layout, lds, T, none, none, twist, hook);
   } else if constexpr (PL::D == 2) {
     if constexpr (STEP == 0)
-      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL, PERSIST>(
-          tid, tile, in, out, krows, cols, layout, lds, T, inv1, none, none, release);
+      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL>(
+          tid, tile, in, out, krows, cols, layout, lds, T, inv1, none, none, hook);
     else if constexpr (STEP == 1)
-      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L1, PL::R1, FULL, PERSIST>(
-          tid, tile, in, out, krows, cols, layout, lds, T, none, fwd1, twist, release);
+      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L1, PL::R1, FULL>(
+          tid, tile, in, out, krows, cols, layout, lds, T, none, fwd1, twist, hook);
     else
-      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL, PERSIST>(
-          tid, tile, in, out, krows, cols, layout, lds, T, none, none, none, release);
+      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL>(
+          tid, tile, in, out, krows, cols, layout, lds, T, none, none, none, hook);
   } else {
     if constexpr (STEP == 0)
-      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL, PERSIST>(
-          tid, tile, in, out, krows, cols, layout, lds, T, inv1, none, none, release);
+      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL>(
+          tid, tile, in, out, krows, cols, layout, lds, T, inv1, none, none, hook);
     else if constexpr (STEP == 1)
-      encode_pass<PL, L2, PL::R2, PASS_INTT, false, false, 0, 0, FULL, PERSIST>(
-          tid, tile, in, out, krows, cols, layout, lds, T, inv2, none, none, release);
+      encode_pass<PL, L2, PL::R2, PASS_INTT, false, false, 0, 0, FULL>(
+          tid, tile, in, out, krows, cols, layout, lds, T, inv2, none, none, hook);
     else if constexpr (STEP == 2)
-      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L2, PL::R2, FULL, PERSIST>(
-          tid, tile, in, out, krows, cols, layout, lds, T, none, fwd2, twist, release);
+      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L2, PL::R2, FULL>(
+          tid, tile, in, out, krows, cols, layout, lds, T, none, fwd2, twist, hook);
     else if constexpr (STEP == 3)
-      encode_pass<PL, L2, PL::R2, PASS_NTT, false, false, L1, PL::R1, FULL, PERSIST>(
-          tid, tile, in, out, krows, cols, layout, lds, T, none, fwd1, none, release);
+      encode_pass<PL, L2, PL::R2, PASS_NTT, false, false, L1, PL::R1, FULL>(
+          tid, tile, in, out, krows, cols, layout, lds, T, none, fwd1, none, hook);
     else
-      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL, PERSIST>(
-          tid, tile, in, out, krows, cols, layout, lds, T, none, none, none, release);
+      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL>(
+          tid, tile, in, out, krows, cols, layout, lds, T, none, none, none, hook);
   }
 }
 
-// Copies tile `tile` of the row-major input into the LDS tile, in the
-// [row][column] layout the passes read (u32, FULL tiles only).  On the GPU
-// each lane moves 16 bytes straight from global memory into LDS
-// (global_load_lds_dwordx4): a wave instruction fills 64 lanes * 16 bytes =
-// 256/W consecutive rows, so the LDS destination is contiguous per wave as
-// the instruction requires.  Completion is tracked by vmcnt.
+// Register pipeline of the first pass for whole tiles in the row layout
+// (plans with D >= 2 and LDS twiddles).  load() brings a tile's first-pass
+// values into registers -- issued while the previous tile is still being
+// transformed, so the loads are in flight behind that work -- first()
+// transforms them into the LDS tile, and store_even() writes them to the even
+// output rows, one group at a time during the next pass, so the stores too
+// are spread over compute instead of issued in one burst.
 template <class PL>
-ZHD void prefetch_tile(int tid, uint32_t tile, const uint32_t* __restrict__ in, uint32_t cols,
-                       int32_t* __restrict__ lds) {
-  constexpr int LPR = PL::W / 4;   // lanes per row
-  constexpr int RPI = 64 / LPR;    // rows per wave instruction
-  constexpr int NWAVE = PL::NT / 64;
-  constexpr int NI = PL::S / RPI;  // wave instructions per tile
-  static_assert(PL::W % 4 == 0 && 64 % LPR == 0 && PL::S % RPI == 0 && NI % NWAVE == 0,
-                "tile shape does not split into whole wave loads");
-  const int wave = tid / 64, lane = tid % 64;
-  const uint32_t c4 = (uint32_t)(lane % LPR) * 4;
+struct Pipe {
+  static constexpr int R = PL::R1, RR = 1 << R, LO = PL::LO1;
+  static constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
+  static_assert(PL::D >= 2 && PL::TWL && (PL::W << (PL::L - R)) % PL::NT == 0, "pipeline shape");
+
+  static ZHD void load(int tid, uint32_t tile, const uint32_t* __restrict__ in, uint32_t cols,
+                       uint32_t (&raw)[G][RR]) {
+    const size_t istep = (size_t)cols << LO;
 #pragma unroll
-  for (int i = 0; i < NI / NWAVE; i++) {
-    const uint32_t row0 = (uint32_t)((wave * (NI / NWAVE) + i) * RPI);
-    const uint32_t j = row0 + (uint32_t)(lane / LPR);
-    const uint32_t* src = in + (size_t)j * cols + (size_t)tile * PL::W + c4;
-#if defined(__HIP_DEVICE_COMPILE__)
-    __builtin_amdgcn_global_load_lds(
-        (const void*)src, (__attribute__((address_space(3))) void*)(lds + row0 * PL::W), 16, 0, 0);
-#else
-    for (int q = 0; q < 4; q++) lds[j * PL::W + c4 + q] = (int32_t)src[q];
-#endif
+    for (int gi = 0; gi < G; gi++) {
+      const uint32_t g = (uint32_t)(gi * PL::NT + tid);
+      const uint32_t c = g % PL::W, u = g / PL::W;
+      const uint32_t* src = in + (size_t)row_of<LO, R>(u, 0) * cols + (size_t)tile * PL::W + c;
+#pragma unroll
+      for (int m = 0; m < RR; m++) raw[gi][m] = src[(size_t)m * istep];
+    }
   }
-}
+
+  static ZHD void store_even(int tid, int gi, uint32_t tile, uint32_t* __restrict__ out,
+                             uint32_t cols, const uint32_t (&r)[RR]) {
+    const size_t ostep = (size_t)cols << (LO + 1);
+    const uint32_t g = (uint32_t)(gi * PL::NT + tid);
+    const uint32_t c = g % PL::W, u = g / PL::W;
+    uint32_t* dst = out + (size_t)(2 * row_of<LO, R>(u, 0)) * cols + (size_t)tile * PL::W + c;
+#pragma unroll
+    for (int m = 0; m < RR; m++) dst[(size_t)m * ostep] = r[m];
+  }
+
+  // The first pass (INTT of the top digit) from registers into the LDS tile.
+  static ZHD void first(int tid, const uint32_t (&raw)[G][RR], int32_t* __restrict__ lds,
+                        const EncodeTabs& T, const TwTab inv1) {
+    const Tw* rinv = T.roots + 64;
+#pragma unroll
+    for (int gi = 0; gi < G; gi++) {
+      group_fence();
+      const uint32_t g = (uint32_t)(gi * PL::NT + tid);
+      const uint32_t c = g % PL::W, u = g / PL::W;
+      double x[RR];
+#pragma unroll
+      for (int m = 0; m < RR; m++) x[m] = (double)raw[gi][m];
+      dft_reg<R>(x, rinv);
+      const uint32_t base = u << R;  // u < 2^LO for the top digit
+      if constexpr (PL::D == 2) {
+#pragma unroll
+        for (int k = 0; k < RR; k++) x[k] = mulmod(x[k], tw_get<true>(inv1, base + k));
+      } else {
+        x[0] = reduce(x[0]);
+#pragma unroll
+        for (int k = 1; k < RR; k++) x[k] = mulmod(x[k], tw_get<true>(inv1, base + k));
+      }
+#pragma unroll
+      for (int m = 0; m < RR; m++) lds[row_of<LO, R>(u, m) * PL::W + c] = (int32_t)x[m];
+    }
+  }
+};
 
 }  // namespace zoda
