@@ -57,6 +57,30 @@ def test_trace_results_equal_alone(zh, torch, mode):
         assert torch.equal(t_out, alone(zh, torch, t_in)), (i, sides[i])
 
 
+def test_burst_of_new_sides(zh, torch):
+    # Requests of sides the server has not seen yet, a few microseconds
+    # apart: consecutive batched launches land on different slot streams, and
+    # each must see the table entry its square needs (uploaded before any
+    # launch names it), whichever stream launched first.
+    rng = np.random.default_rng(21)
+    sides = [int(s) for s in rng.choice([16, 32, 64, 128], size=24)]
+    ins, outs, specs = [], [], []
+    for i, s in enumerate(sides):
+        t_in = torch.empty((s, 128 * s), dtype=torch.int32, device="cuda")
+        zh.fill_uniform(t_in, seed=700 + i)
+        t_out = torch.zeros((2 * s, 128 * s), dtype=torch.int32, device="cuda")
+        ins.append(t_in)
+        outs.append(t_out)
+        specs.append((t_in.data_ptr(), t_out.data_ptr(), s, 128 * s, s))
+    torch.cuda.synchronize()
+    arrivals = np.arange(len(sides), dtype=np.float64) * 5.0
+    print(f"[case] serve burst sides={sides}", flush=True)
+    lat, stats = zh.serve_trace(specs, arrivals, hbm_budget=200 * 10**9, slots=4, mode=0)
+    assert (lat > 0).all() and stats["rejected"] == 0
+    for i, (t_in, t_out) in enumerate(zip(ins, outs)):
+        assert torch.equal(t_out, alone(zh, torch, t_in)), (i, sides[i])
+
+
 def test_budget_bounds_admission(zh, torch):
     sides, arrivals, ins, outs, specs = trace(zh, torch, 24, 20000, seed=9, sides=(64, 256, 512))
     biggest = max((s + 2 * s) * 128 * s * 4 for s in sides)

@@ -40,30 +40,55 @@ __device__ __attribute__((noinline)) void batch_tile_steps(int tid, uint32_t til
 
 __global__ void batch_reset(uint32_t* counter) { *counter = 0u; }
 
+// All passes of column tile `tile` of square d with the batched plan of its side.
+__device__ __forceinline__ void run_batch_tile(int tid, const SquareDesc& d, uint32_t tile,
+                                               int32_t* lds, const EncodeTabs& T) {
+  with_batch_plan((int)d.L, (int)d.variant, [&](auto plan) -> int {
+    using PL = decltype(plan);
+    if (d.full)
+      batch_tile_steps<PL, true>(tid, tile, d, lds, T);
+    else
+      batch_tile_steps<PL, false>(tid, tile, d, lds, T);
+    return 0;
+  });
+}
+
+// Next work item of the work-group (one atomic per work-group, broadcast
+// through LDS); uniform across the work-group.
+__device__ __forceinline__ uint32_t take_item(int tid, uint32_t* counter, uint32_t base) {
+  __shared__ uint32_t s_item;
+  if (tid == 0) s_item = atomicAdd(counter, 1u) - base;
+  __syncthreads();
+  const uint32_t it = s_item;
+  __syncthreads();
+  return it;
+}
+
 __global__ __launch_bounds__(BATCH_NT) void batch_kernel(const SquareDesc* __restrict__ squares,
                                                          const BatchItem* __restrict__ items,
                                                          uint32_t nitems, uint32_t* counter,
                                                          const EncodeTabs* __restrict__ tabs) {
   extern __shared__ int32_t lds[];
-  __shared__ uint32_t s_item;
   const int tid = (int)threadIdx.x;
   for (;;) {
-    if (tid == 0) s_item = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t it = s_item;
-    __syncthreads();
-    if (it >= nitems) break;  // uniform: every wave sees the same item
+    const uint32_t it = take_item(tid, counter, 0);
+    if (it >= nitems) break;
     const BatchItem bi = items[it];
     const SquareDesc d = squares[bi.square];
-    const EncodeTabs T = tabs[d.L * BATCH_VARIANTS + d.variant];
-    with_batch_plan((int)d.L, (int)d.variant, [&](auto plan) -> int {
-      using PL = decltype(plan);
-      if (d.full)
-        batch_tile_steps<PL, true>(tid, bi.tile, d, lds, T);
-      else
-        batch_tile_steps<PL, false>(tid, bi.tile, d, lds, T);
-      return 0;
-    });
+    run_batch_tile(tid, d, bi.tile, lds, tabs[d.L * BATCH_VARIANTS + d.variant]);
+  }
+}
+
+__global__ __launch_bounds__(BATCH_NT) void serve_batch_kernel(const ServeBatch b) {
+  extern __shared__ int32_t lds[];
+  const int tid = (int)threadIdx.x;
+  for (;;) {
+    const uint32_t it = take_item(tid, b.counter, b.base);
+    if (it >= b.items) break;
+    uint32_t q = 0;
+    while (q + 1 < b.n && b.first[q + 1] <= it) q++;
+    const SquareDesc d = b.sq[q];
+    run_batch_tile(tid, d, it - b.first[q], lds, b.tabs[d.L * BATCH_VARIANTS + d.variant]);
   }
 }
 
@@ -84,6 +109,14 @@ int launch_batch_items(const SquareDesc* d_squares, const BatchItem* d_items, ui
   hipLaunchKernelGGL(batch_kernel, dim3(std::min(grid, nitems)), dim3(BATCH_NT), BATCH_LDS_WORDS * 4,
                      stream, d_squares, d_items, nitems, d_counter, d_tabs);
   ZODA_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_serve_batch(const ServeBatch& b, uint32_t grid, hipStream_t stream, uint32_t* advance) {
+  const uint32_t g = std::min(grid, b.items);
+  hipLaunchKernelGGL(serve_batch_kernel, dim3(g), dim3(BATCH_NT), BATCH_LDS_WORDS * 4, stream, b);
+  ZODA_HIP(hipGetLastError());
+  *advance = b.items + g;
   return 0;
 }
 

@@ -19,6 +19,21 @@ struct BatchItem {
   uint32_t square, tile;
 };
 
+// A batch whose descriptors travel in the kernel arguments (serving loop):
+// square q owns work items [first[q], first[q + 1]).  The work counter is
+// never reset: a launch takes items counter - base, and the host advances
+// `base` by items + work-groups after each launch on the same counter, which
+// is exactly how far that launch moves it (every work-group ends on one take
+// past the last item).
+constexpr uint32_t SERVE_MAX_BATCH = 48;
+struct ServeBatch {
+  SquareDesc sq[SERVE_MAX_BATCH];
+  uint32_t first[SERVE_MAX_BATCH + 1];
+  uint32_t n, items, base;
+  uint32_t* counter;
+  const EncodeTabs* tabs;
+};
+
 int check_batch_square(const zoda_square& s, uint32_t i);
 // Fills *d for square s with the device's autotuned variant; builds the
 // (side, variant) table pointers into tabs_by_slot[L * 2 + variant] once.
@@ -30,5 +45,9 @@ int batch_grid(DeviceState* st, uint32_t* grid);
 int launch_batch_items(const SquareDesc* d_squares, const BatchItem* d_items, uint32_t nitems,
                        uint32_t* d_counter, const EncodeTabs* d_tabs, uint32_t grid,
                        hipStream_t stream);
+
+// Persistent kernel over a ServeBatch on `stream`, on min(grid, items)
+// work-groups; *advance receives how far it moves the counter.
+int launch_serve_batch(const ServeBatch& b, uint32_t grid, hipStream_t stream, uint32_t* advance);
 
 }  // namespace zoda

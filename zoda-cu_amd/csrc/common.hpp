@@ -72,6 +72,9 @@ struct DeviceState {
   std::vector<void*> buffers;
   std::vector<size_t> buffer_bytes;
   std::vector<hipStream_t> streams;
+  // Per-kernel launch set-up done once: persistent grid size (0: the LDS
+  // limit is raised, no grid needed).
+  std::map<const void*, uint32_t> kernel_setup;
 };
 
 // State of the calling thread's current device (created on first use).

@@ -120,6 +120,31 @@ int get_encode_tables(DeviceState* st, int L, int r1, int r2, int rd, const Enco
   return 0;
 }
 
+// Once per device and kernel: raise the dynamic LDS limit when the kernel
+// needs more than the default 64 KiB and, for a persistent kernel, size its
+// grid to fill every CU (*grid; 0 otherwise).
+static int kernel_setup(DeviceState* st, const void* fn, int nt, size_t lds_bytes, bool persistent,
+                        uint32_t* grid) {
+  std::lock_guard<std::recursive_mutex> lock(st->mu);
+  auto it = st->kernel_setup.find(fn);
+  if (it != st->kernel_setup.end()) {
+    *grid = it->second;
+    return 0;
+  }
+  if (lds_bytes > 64 * 1024)
+    ZODA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+  uint32_t g = 0;
+  if (persistent) {
+    int per_cu = 0, cus = 0;
+    ZODA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nt, lds_bytes));
+    ZODA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, st->device));
+    g = (uint32_t)std::max(1, per_cu * cus);
+  }
+  st->kernel_setup.emplace(fn, g);
+  *grid = g;
+  return 0;
+}
+
 template <class PL>
 static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols,
                        uint32_t* d_out, int layout, hipStream_t stream) {
@@ -134,12 +159,9 @@ static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32
   if constexpr (PL::PERSIST) {
     if (full && row) {
       const void* pfn = (const void*)encode_pipelined<PL>;
-      ZODA_HIP(hipFuncSetAttribute(pfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
-      int per_cu = 0, cus = 0;
-      ZODA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_pipelined<PL>, PL::NT, lds_bytes));
-      ZODA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, st->device));
+      uint32_t grid = 0;
+      ZODA_TRY(kernel_setup(st, pfn, PL::NT, lds_bytes, true, &grid));
       uint32_t ntiles = cols / PL::W;
-      uint32_t grid = (uint32_t)std::max(1, per_cu * cus);
       if (grid > ntiles) grid = ntiles;
       void* pargs[] = {(void*)&d_in, (void*)&d_out, (void*)&cols, (void*)&ntiles, (void*)&T};
       ZODA_HIP(hipLaunchKernel(pfn, dim3(grid), dim3(PL::NT), pargs, lds_bytes, stream));
@@ -148,8 +170,8 @@ static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32
   }
   const void* fn = full ? (row ? (const void*)encode_kernel<PL, true, 0> : (const void*)encode_kernel<PL, true, 1>)
                         : (row ? (const void*)encode_kernel<PL, false, 0> : (const void*)encode_kernel<PL, false, 1>);
-  if (lds_bytes > 64 * 1024)
-    ZODA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+  uint32_t unused = 0;
+  ZODA_TRY(kernel_setup(st, fn, PL::NT, lds_bytes, false, &unused));
   const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
   void* args[] = {(void*)&d_in, (void*)&d_out, (void*)&k, (void*)&cols, (void*)&T};
   ZODA_HIP(hipLaunchKernel(fn, dim3(ntiles), dim3(PL::NT), args, lds_bytes, stream));

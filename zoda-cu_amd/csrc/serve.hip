@@ -1,15 +1,19 @@
 // Serving loop: continuous batching of encode requests on one device.
 //
 // Requests (device-resident squares, as zoda_square) queue in arrival order.
-// Each of `slots` launch slots owns a stream, a completion event and
-// preallocated device work-queue arrays.  Whenever a slot is free, every
-// queued request whose in+out bytes fit the HBM budget -- next to the
-// requests still on the GPU -- is admitted and the whole group goes out as one
-// persistent batched launch on that slot's stream.  Requests that arrive while
-// other batches run are admitted as soon as a slot and memory free up, and
-// their launch overlaps the ones in flight.  Twiddle tables stay resident in
-// the per-device cache.  Completion times come from the slot's event,
-// measured on the GPU against an event recorded at start.
+// Each of `slots` launch slots owns a stream, a completion event and a work
+// counter.  Whenever a slot is free it takes the next launch:
+//   * a large request (side >= 2^SERVE_BIG_LOG: its own fused kernel fills the
+//     device) at the head of the queue goes alone through the single-square
+//     kernel; at most slots - 1 of them run at once, in arrival order;
+//   * otherwise every queued small request that fits the HBM budget next to
+//     the requests on the GPU goes out as one persistent batched launch whose
+//     descriptors travel in the kernel arguments (nothing is copied before
+//     the launch; the counter is never reset).
+// Requests that arrive while others run are admitted as soon as a slot and
+// memory free up, and their launches overlap the ones in flight.  Twiddle
+// tables stay resident in the per-device cache.  Completion times come from
+// the slot's event, measured on the GPU against an event recorded at start.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -26,8 +30,7 @@ namespace zoda {
 
 namespace {
 
-constexpr uint32_t SERVE_MAX_BATCH = 64;
-constexpr uint32_t SERVE_MAX_ITEMS = 1u << 20;
+constexpr uint32_t SERVE_BIG_LOG = 8;
 
 double now_us() {
   using namespace std::chrono;
@@ -44,20 +47,19 @@ struct Request {
   double arrival_us = 0;   // host clock
   double done_us = -1;     // host clock, from the GPU event
   uint64_t bytes = 0;
+  bool big = false;
   int state = 0;           // 0 queued, 1 on the GPU, 2 done, 3 rejected
 };
 
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  SquareDesc* d_squares = nullptr;
-  BatchItem* d_items = nullptr;
-  uint32_t* d_counter = nullptr;
-  SquareDesc* h_squares = nullptr;  // pinned staging
-  BatchItem* h_items = nullptr;
+  uint32_t* d_counter = nullptr;    // work counter of this slot's batched launches
+  uint32_t base = 0;                // its value when the next launch starts
   std::vector<uint64_t> members;    // request ids in this launch
   uint64_t bytes = 0;
   bool busy = false;
+  bool big = false;
 };
 
 }  // namespace
@@ -73,20 +75,21 @@ class Server {
     tabs_.assign((BATCH_MAX_LOG + 1) * BATCH_VARIANTS, EncodeTabs{});
     have_.reset(new bool[tabs_.size()]());
     ZODA_HIP(hipMalloc((void**)&d_tabs_, tabs_.size() * sizeof(EncodeTabs)));
+    ZODA_HIP(hipMemset(d_tabs_, 0, tabs_.size() * sizeof(EncodeTabs)));
     ZODA_HIP(hipEventCreate(&start_));
     slots_.resize(nslots);
     for (Slot& s : slots_) {
       ZODA_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
       ZODA_HIP(hipEventCreate(&s.done));
-      if (hipMalloc((void**)&s.d_squares, SERVE_MAX_BATCH * sizeof(SquareDesc)) != hipSuccess ||
-          hipMalloc((void**)&s.d_items, SERVE_MAX_ITEMS * sizeof(BatchItem)) != hipSuccess ||
-          hipMalloc((void**)&s.d_counter, sizeof(uint32_t)) != hipSuccess ||
-          hipHostMalloc((void**)&s.h_squares, SERVE_MAX_BATCH * sizeof(SquareDesc), hipHostMallocDefault) != hipSuccess ||
-          hipHostMalloc((void**)&s.h_items, SERVE_MAX_ITEMS * sizeof(BatchItem), hipHostMallocDefault) != hipSuccess) {
+      if (hipMalloc((void**)&s.d_counter, sizeof(uint32_t)) != hipSuccess) {
         (void)hipGetLastError();
         return fail(ZODA_ENOMEM, "serving slot allocation failed");
       }
+      ZODA_HIP(hipMemset(s.d_counter, 0, sizeof(uint32_t)));
     }
+    // The slot streams do not wait on the null stream: the zero fills must
+    // be complete before the first launch on any of them.
+    ZODA_HIP(hipDeviceSynchronize());
     // Reference point of the GPU clock: the host time right after `start_` completes.
     ZODA_HIP(hipEventRecord(start_, slots_[0].stream));
     ZODA_HIP(hipEventSynchronize(start_));
@@ -102,9 +105,13 @@ class Server {
     r.sq = sq;
     r.arrival_us = arrival_us;
     r.bytes = request_bytes(sq);
-    SquareDesc d;
-    ZODA_TRY(describe_square(st_, sq, &d, tabs_.data(), have_.get()));
-    if (r.bytes > budget_ || d.tiles > SERVE_MAX_ITEMS) r.state = 3;  // could never be admitted
+    r.big = log2_ceil(sq.k) >= SERVE_BIG_LOG;
+    if (!r.big) {  // resolve the batched plan and its tables now, off the launch path
+      SquareDesc d;
+      ZODA_TRY(describe_square(st_, sq, &d, tabs_.data(), have_.get()));
+      ZODA_TRY(sync_tables());
+    }
+    if (r.bytes > budget_) r.state = 3;  // could never be admitted
     reqs_.push_back(r);
     *id = reqs_.size() - 1;
     if (r.state == 0) queue_.push_back(*id);
@@ -129,13 +136,18 @@ class Server {
         completed_++;
       }
       in_flight_bytes_ -= s.bytes;
+      if (s.big) big_in_flight_--;
       s.members.clear();
       s.bytes = 0;
       s.busy = false;
+      s.big = false;
     }
     for (Slot& s : slots_) {
-      if (s.busy || queue_.empty()) continue;
-      ZODA_TRY(launch_slot(s));
+      if (s.busy) continue;
+      if (queue_.empty()) break;
+      bool launched = false;
+      ZODA_TRY(fill_slot(s, &launched));
+      if (!launched) break;
     }
     return 0;
   }
@@ -205,60 +217,93 @@ class Server {
   }
 
  private:
-  int launch_slot(Slot& s) {
-    uint32_t nsq = 0, nitems = 0;
-    uint64_t bytes = 0;
-    std::vector<uint64_t> taken;
-    // FIFO admission: stop at the first request that does not fit.
-    while (!queue_.empty() && nsq < SERVE_MAX_BATCH) {
-      const uint64_t id = queue_.front();
-      Request& r = reqs_[id];
-      if (in_flight_bytes_ + bytes + r.bytes > budget_) break;
-      SquareDesc d;
-      ZODA_TRY(describe_square(st_, r.sq, &d, tabs_.data(), have_.get()));
-      if (nitems + d.tiles > SERVE_MAX_ITEMS) break;
-      s.h_squares[nsq] = d;
-      nitems += d.tiles;
-      bytes += r.bytes;
-      taken.push_back(id);
-      queue_.pop_front();
-      nsq++;
+  // The next launch for free slot s (see the top of this file).  Large
+  // requests keep arrival order; small ones may pass a large head that waits
+  // for a slot or memory, but not into the memory the head needs.
+  int fill_slot(Slot& s, bool* launched) {
+    *launched = false;
+    uint64_t reserve = 0;
+    const uint64_t head = queue_.front();
+    if (reqs_[head].big) {
+      Request& r = reqs_[head];
+      const bool slot_free = slots_.size() == 1 || big_in_flight_ + 1 < slots_.size();
+      if (slot_free && in_flight_bytes_ + r.bytes <= budget_) {
+        const zoda_square& q = r.sq;
+        ZODA_TRY(launch_encode(st_, q.in, q.k, q.cols, q.n, q.out, ZODA_LAYOUT_ROW, s.stream));
+        queue_.pop_front();
+        s.big = true;
+        big_in_flight_++;
+        return commit(s, std::vector<uint64_t>{head}, r.bytes, launched);
+      }
+      reserve = r.bytes;
     }
-    if (nsq == 0) return 0;
-    // Largest sides first inside the launch.
-    std::vector<uint32_t> order(nsq);
-    for (uint32_t i = 0; i < nsq; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return s.h_squares[a].L > s.h_squares[b].L; });
-    uint32_t n = 0;
-    for (uint32_t i : order)
-      for (uint32_t t = 0; t < s.h_squares[i].tiles; t++) s.h_items[n++] = BatchItem{i, t};
-    if (tabs_dirty()) ZODA_HIP(hipMemcpyAsync(d_tabs_, tabs_.data(), tabs_.size() * sizeof(EncodeTabs),
-                                              hipMemcpyHostToDevice, s.stream));
-    ZODA_HIP(hipMemcpyAsync(s.d_squares, s.h_squares, nsq * sizeof(SquareDesc), hipMemcpyHostToDevice, s.stream));
-    ZODA_HIP(hipMemcpyAsync(s.d_items, s.h_items, n * sizeof(BatchItem), hipMemcpyHostToDevice, s.stream));
-    ZODA_TRY(launch_batch_items(s.d_squares, s.d_items, n, s.d_counter, d_tabs_, grid_, s.stream));
+    ServeBatch b{};
+    std::vector<uint64_t> taken;
+    uint64_t bytes = 0;
+    for (auto it = queue_.begin(); it != queue_.end() && taken.size() < SERVE_MAX_BATCH;) {
+      Request& r = reqs_[*it];
+      if (r.big) {
+        ++it;
+        continue;
+      }
+      if (in_flight_bytes_ + reserve + bytes + r.bytes > budget_) break;
+      bytes += r.bytes;
+      taken.push_back(*it);
+      it = queue_.erase(it);
+    }
+    if (taken.empty()) return 0;
+    // Largest sides first inside the launch: long tiles start early, short
+    // ones fill the tail.
+    std::stable_sort(taken.begin(), taken.end(), [&](uint64_t x, uint64_t y) {
+      return reqs_[x].sq.k > reqs_[y].sq.k;
+    });
+    uint64_t items = 0;
+    for (uint32_t q = 0; q < taken.size(); q++) {
+      ZODA_TRY(describe_square(st_, reqs_[taken[q]].sq, &b.sq[q], tabs_.data(), have_.get()));
+      b.first[q] = (uint32_t)items;
+      items += b.sq[q].tiles;
+    }
+    if (items >= (1ull << 31)) return fail(ZODA_EINVAL, "batch of %llu tiles is too large", (unsigned long long)items);
+    b.n = (uint32_t)taken.size();
+    b.first[b.n] = (uint32_t)items;
+    b.items = (uint32_t)items;
+    b.base = s.base;
+    b.counter = s.d_counter;
+    b.tabs = d_tabs_;
+    ZODA_TRY(sync_tables());
+    uint32_t advance = 0;
+    ZODA_TRY(launch_serve_batch(b, grid_, s.stream, &advance));
+    s.base += advance;
+    return commit(s, std::move(taken), bytes, launched);
+  }
+
+  int commit(Slot& s, std::vector<uint64_t> members, uint64_t bytes, bool* launched) {
     ZODA_HIP(hipEventRecord(s.done, s.stream));
-    for (uint64_t id : taken) reqs_[id].state = 1;
-    s.members = std::move(taken);
+    for (uint64_t id : members) reqs_[id].state = 1;
+    max_batch_ = std::max(max_batch_, (uint32_t)members.size());
+    s.members = std::move(members);
     s.bytes = bytes;
     s.busy = true;
     in_flight_bytes_ += bytes;
     peak_bytes_ = std::max(peak_bytes_, in_flight_bytes_);
     batches_++;
-    max_batch_ = std::max(max_batch_, nsq);
+    *launched = true;
     return 0;
   }
 
-  // The table pointer array only grows; upload it when a new (side, variant)
-  // appeared.  Pinned staging is not needed: it changes rarely and the copy
-  // is stream-ordered before the launch that reads it.
-  bool tabs_dirty() {
+  // The device copy of the table pointers, indexed by (side, variant), only
+  // grows.  A new entry is uploaded synchronously, before any launch that
+  // names it: launches on every slot stream read the array, and a copy
+  // ordered on one stream would not be ordered before a kernel on another.
+  // Entries already on the device are rewritten with the same bytes, so a
+  // kernel reading them meanwhile is unaffected.
+  int sync_tables() {
     size_t n = 0;
     for (size_t i = 0; i < tabs_.size(); i++) n += have_[i];
-    if (n == tabs_uploaded_) return false;
+    if (n == tabs_uploaded_) return 0;
+    ZODA_HIP(hipMemcpy(d_tabs_, tabs_.data(), tabs_.size() * sizeof(EncodeTabs), hipMemcpyHostToDevice));
     tabs_uploaded_ = n;
-    return true;
+    return 0;
   }
 
   void release() {
@@ -267,11 +312,7 @@ class Server {
       if (s.stream) (void)hipStreamSynchronize(s.stream);
       if (s.done) (void)hipEventDestroy(s.done);
       if (s.stream) (void)hipStreamDestroy(s.stream);
-      (void)hipFree(s.d_squares);
-      (void)hipFree(s.d_items);
       (void)hipFree(s.d_counter);
-      if (s.h_squares) (void)hipHostFree(s.h_squares);
-      if (s.h_items) (void)hipHostFree(s.h_items);
     }
     slots_.clear();
     if (start_) (void)hipEventDestroy(start_);
@@ -283,6 +324,7 @@ class Server {
   DeviceState* st_ = nullptr;
   uint64_t budget_ = 0, in_flight_bytes_ = 0, peak_bytes_ = 0;
   uint32_t grid_ = 0, batches_ = 0, max_batch_ = 0, rejected_ = 0;
+  size_t big_in_flight_ = 0;
   uint64_t completed_ = 0;
   std::vector<EncodeTabs> tabs_;
   std::unique_ptr<bool[]> have_;
@@ -415,9 +457,9 @@ extern "C" int zoda_hip_serve_trace(const zoda_square* reqs, const double* arriv
   for (auto& e : ev)
     if (!rc && hipEventCreate(&e) != hipSuccess) rc = fail(ZODA_ENOMEM, "event creation failed");
   double start_host = 0, t0 = 0;
+  if (!rc && (hipEventRecord(ev[count], s) != hipSuccess || hipEventSynchronize(ev[count]) != hipSuccess))
+    rc = fail(ZODA_ESTATE, "start event failed");
   if (!rc) {
-    hipEventRecord(ev[count], s);
-    hipEventSynchronize(ev[count]);
     start_host = now_us();
     t0 = start_host;
   }
@@ -434,7 +476,10 @@ extern "C" int zoda_hip_serve_trace(const zoda_square* reqs, const double* arriv
   double last = 0;
   for (uint32_t i = 0; i < count && !rc; i++) {
     float ms = 0;
-    hipEventElapsedTime(&ms, ev[count], ev[i]);
+    if (hipEventElapsedTime(&ms, ev[count], ev[i]) != hipSuccess) {
+      rc = fail(ZODA_ESTATE, "event timing failed");
+      break;
+    }
     const double done = start_host + ms * 1000.0;
     latency_us[i] = done - (t0 + arrival_us[i]);
     last = std::max(last, done);
