@@ -92,6 +92,22 @@ def test_graph_replay(zh, torch):
     plan.destroy()
 
 
+@pytest.mark.parametrize("sides", [[512, 256, 256], [32, 64, 128]])
+def test_graph_replay_one_kind(zh, torch, sides):
+    # only squares that run as their own fused launches, or only queued ones
+    ins, outs, specs = make_batch(zh, torch, sides, seed=4)
+    plan = zh.BatchPlan(specs)
+    stream = torch.cuda.current_stream().cuda_stream
+    for rep in range(2):
+        for t_out in outs:
+            t_out.zero_()
+        plan.launch(stream)
+        torch.cuda.synchronize()
+        for i, (t_in, t_out, s) in enumerate(zip(ins, outs, sides)):
+            assert torch.equal(t_out, plain(zh, torch, t_in, s)), (rep, i, s)
+    plan.destroy()
+
+
 def test_autotune_variants_agree(zh, torch):
     tuned = zh.autotune([32, 64, 128, 256, 512, 1024], iters=2)
     print("[autotune]", tuned, flush=True)

@@ -22,7 +22,13 @@ namespace zoda {
 // VGPR budget is the largest plan's rather than a merge of all of them.
 template <class PL, bool FULL>
 __device__ __attribute__((noinline)) void batch_tile_steps(int tid, uint32_t tile, const SquareDesc& d,
-                                                 int32_t* lds, const EncodeTabs& T) {
+                                                 int32_t* lds, const EncodeTabs& T, bool load) {
+  if constexpr (PL::TWL) {
+    if (load) {  // uniform: the work-group changed plan since its last tile
+      encode_load_tables<PL>(tid, lds, T);
+      __syncthreads();
+    }
+  }
   encode_step<PL, 0, FULL>(tid, tile, d.in, d.out, d.k, d.cols, 0, lds, T);
   if constexpr (PL::STEPS > 1) {
     __syncthreads();
@@ -40,15 +46,21 @@ __device__ __attribute__((noinline)) void batch_tile_steps(int tid, uint32_t til
 
 __global__ void batch_reset(uint32_t* counter) { *counter = 0u; }
 
-// All passes of column tile `tile` of square d with the batched plan of its side.
+// All passes of column tile `tile` of square d with the batched plan of its
+// side.  *loaded names the plan whose twiddle tables are in LDS (-1: none);
+// they are reloaded when the plan changes.  The caller's barrier after the
+// previous tile separates its last reads of LDS from the reload.
 __device__ __forceinline__ void run_batch_tile(int tid, const SquareDesc& d, uint32_t tile,
-                                               int32_t* lds, const EncodeTabs& T) {
+                                               int32_t* lds, const EncodeTabs& T, int* loaded) {
+  const int key = (int)(d.L * BATCH_VARIANTS + d.variant);
+  const bool load = key != *loaded;
+  *loaded = key;
   with_batch_plan((int)d.L, (int)d.variant, [&](auto plan) -> int {
     using PL = decltype(plan);
     if (d.full)
-      batch_tile_steps<PL, true>(tid, tile, d, lds, T);
+      batch_tile_steps<PL, true>(tid, tile, d, lds, T, load);
     else
-      batch_tile_steps<PL, false>(tid, tile, d, lds, T);
+      batch_tile_steps<PL, false>(tid, tile, d, lds, T, load);
     return 0;
   });
 }
@@ -70,30 +82,37 @@ __global__ __launch_bounds__(BATCH_NT) void batch_kernel(const SquareDesc* __res
                                                          const EncodeTabs* __restrict__ tabs) {
   extern __shared__ int32_t lds[];
   const int tid = (int)threadIdx.x;
+  int loaded = -1;
   for (;;) {
     const uint32_t it = take_item(tid, counter, 0);
     if (it >= nitems) break;
     const BatchItem bi = items[it];
     const SquareDesc d = squares[bi.square];
-    run_batch_tile(tid, d, bi.tile, lds, tabs[d.L * BATCH_VARIANTS + d.variant]);
+    run_batch_tile(tid, d, bi.tile, lds, tabs[d.L * BATCH_VARIANTS + d.variant], &loaded);
   }
 }
 
 __global__ __launch_bounds__(BATCH_NT) void serve_batch_kernel(const ServeBatch b) {
   extern __shared__ int32_t lds[];
   const int tid = (int)threadIdx.x;
+  int loaded = -1;
   for (;;) {
     const uint32_t it = take_item(tid, b.counter, b.base);
     if (it >= b.items) break;
     uint32_t q = 0;
     while (q + 1 < b.n && b.first[q + 1] <= it) q++;
     const SquareDesc d = b.sq[q];
-    run_batch_tile(tid, d, it - b.first[q], lds, b.tabs[d.L * BATCH_VARIANTS + d.variant]);
+    run_batch_tile(tid, d, it - b.first[q], lds, b.tabs[d.L * BATCH_VARIANTS + d.variant], &loaded);
   }
 }
 
 int batch_grid(DeviceState* st, uint32_t* grid) {
   int per_cu = 0, cus = 0;
+  // Both batched kernels take more than the default 64 KiB of dynamic LDS.
+  ZODA_HIP(hipFuncSetAttribute((const void*)batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               BATCH_LDS_WORDS * 4));
+  ZODA_HIP(hipFuncSetAttribute((const void*)serve_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               BATCH_LDS_WORDS * 4));
   ZODA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel, BATCH_NT,
                                                          BATCH_LDS_WORDS * 4));
   ZODA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, st->device));
@@ -165,6 +184,11 @@ struct zoda_batch_plan {
   EncodeTabs* d_tabs = nullptr;
   uint32_t* d_counter = nullptr;
   uint32_t nitems = 0, grid = 0;
+  // Squares that run as their own fused launches, largest first, on a branch
+  // (side stream, fork/join events) parallel to the work-queue kernel.
+  std::vector<zoda_square> direct;
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -173,6 +197,9 @@ static void free_plan(zoda_batch_plan* p) {
   if (!p) return;
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->fork) (void)hipEventDestroy(p->fork);
+  if (p->join) (void)hipEventDestroy(p->join);
+  if (p->side) (void)hipStreamDestroy(p->side);
   (void)hipFree(p->d_squares);
   (void)hipFree(p->d_items);
   (void)hipFree(p->d_tabs);
@@ -180,14 +207,35 @@ static void free_plan(zoda_batch_plan* p) {
   delete p;
 }
 
+// [queue reset, persistent kernel] on `stream`, the direct squares' fused
+// kernels on the side branch, joined back into `stream`.
 static int launch_batch_direct(zoda_batch_plan* p, hipStream_t stream) {
-  return launch_batch_items(p->d_squares, p->d_items, p->nitems, p->d_counter, p->d_tabs, p->grid,
-                            stream);
+  DeviceState* st = nullptr;
+  ZODA_TRY(current_state(&st));
+  const bool queue = p->nitems > 0;
+  hipStream_t ds = queue ? p->side : stream;
+  if (queue && !p->direct.empty()) {
+    ZODA_HIP(hipEventRecord(p->fork, stream));
+    ZODA_HIP(hipStreamWaitEvent(ds, p->fork, 0));
+  }
+  for (const zoda_square& q : p->direct)
+    ZODA_TRY(launch_encode(st, q.in, q.k, q.cols, q.n, q.out, ZODA_LAYOUT_ROW, ds));
+  if (queue)
+    ZODA_TRY(launch_batch_items(p->d_squares, p->d_items, p->nitems, p->d_counter, p->d_tabs, p->grid,
+                                stream));
+  if (queue && !p->direct.empty()) {
+    ZODA_HIP(hipEventRecord(p->join, ds));
+    ZODA_HIP(hipStreamWaitEvent(stream, p->join, 0));
+  }
+  return 0;
 }
 
 // Builds descriptors, the largest-first item list and the per-(side, variant)
-// table pointers on the device; with `graph`, also captures [reset, kernel].
-static int build_plan(const zoda_square* squares, uint32_t count, bool graph, zoda_batch_plan** out) {
+// table pointers on the device; with `graph`, also captures the launch.  With
+// allow_direct, squares of side >= 2^DIRECT_MIN_LOG run as their own fused
+// launches beside the queue (the autotuner times the queue alone).
+static int build_plan(const zoda_square* squares, uint32_t count, bool graph, bool allow_direct,
+                      zoda_batch_plan** out) {
   clear_error();
   if (!out) return fail(ZODA_EINVAL, "plan pointer is NULL");
   *out = nullptr;
@@ -197,47 +245,67 @@ static int build_plan(const zoda_square* squares, uint32_t count, bool graph, zo
   ZODA_TRY(current_state(&st));
   std::lock_guard<std::recursive_mutex> lock(st->mu);
 
-  std::vector<SquareDesc> descs(count);
+  std::vector<zoda_square> direct, queued;
+  for (uint32_t i = 0; i < count; i++)
+    (allow_direct && log2_ceil(squares[i].k) >= DIRECT_MIN_LOG ? direct : queued).push_back(squares[i]);
+  std::stable_sort(direct.begin(), direct.end(),
+                   [](const zoda_square& a, const zoda_square& b) { return a.k > b.k; });
+  std::vector<SquareDesc> descs(queued.size());
   std::vector<EncodeTabs> tabs((BATCH_MAX_LOG + 1) * BATCH_VARIANTS, EncodeTabs{});
   std::unique_ptr<bool[]> have(new bool[tabs.size()]());
   uint64_t nitems = 0;
-  for (uint32_t i = 0; i < count; i++) {
-    ZODA_TRY(describe_square(st, squares[i], &descs[i], tabs.data(), have.get()));
+  for (size_t i = 0; i < queued.size(); i++) {
+    ZODA_TRY(describe_square(st, queued[i], &descs[i], tabs.data(), have.get()));
     nitems += descs[i].tiles;
   }
   if (nitems >= (1ull << 32)) return fail(ZODA_EINVAL, "batch of %llu tiles is too large", (unsigned long long)nitems);
   // Largest sides first so the long tiles start early and the short ones fill the tail.
-  std::vector<uint32_t> order(count);
-  for (uint32_t i = 0; i < count; i++) order[i] = i;
+  std::vector<uint32_t> order(queued.size());
+  for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(),
                    [&](uint32_t a, uint32_t b) { return descs[a].L > descs[b].L; });
   std::vector<BatchItem> items;
   items.reserve(nitems);
   for (uint32_t i : order)
     for (uint32_t t = 0; t < descs[i].tiles; t++) items.push_back(BatchItem{i, t});
+  // One-time set-up of the direct squares' kernels now: not allowed while
+  // capturing.
+  for (const zoda_square& q : direct)
+    ZODA_TRY(launch_encode(st, q.in, q.k, q.cols, q.n, q.out, ZODA_LAYOUT_ROW, nullptr, true));
 
   zoda_batch_plan* p = new zoda_batch_plan();
   p->device = st->device;
   p->nitems = (uint32_t)items.size();
+  p->direct = std::move(direct);
   auto bail = [&](int rc) {
     free_plan(p);
     return rc;
   };
-  if (hipMalloc((void**)&p->d_squares, descs.size() * sizeof(SquareDesc)) != hipSuccess ||
-      hipMalloc((void**)&p->d_items, items.size() * sizeof(BatchItem)) != hipSuccess ||
+  if (hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->join, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    return bail(fail(ZODA_ENOMEM, "batch plan stream/event creation failed"));
+  }
+  // Zero-sized arrays still get an allocation so the kernel arguments are valid.
+  if (hipMalloc((void**)&p->d_squares, std::max<size_t>(1, descs.size()) * sizeof(SquareDesc)) != hipSuccess ||
+      hipMalloc((void**)&p->d_items, std::max<size_t>(1, items.size()) * sizeof(BatchItem)) != hipSuccess ||
       hipMalloc((void**)&p->d_tabs, tabs.size() * sizeof(EncodeTabs)) != hipSuccess ||
       hipMalloc((void**)&p->d_counter, sizeof(uint32_t)) != hipSuccess) {
     (void)hipGetLastError();
     return bail(fail(ZODA_ENOMEM, "batch plan allocation failed"));
   }
-  hipError_t e = hipMemcpy(p->d_squares, descs.data(), descs.size() * sizeof(SquareDesc), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(p->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
+  hipError_t e = hipSuccess;
+  if (!descs.empty())
+    e = hipMemcpy(p->d_squares, descs.data(), descs.size() * sizeof(SquareDesc), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !items.empty())
+    e = hipMemcpy(p->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_tabs, tabs.data(), tabs.size() * sizeof(EncodeTabs), hipMemcpyHostToDevice);
   if (e != hipSuccess) return bail(fail((int)e, "batch plan upload failed: %s", hipGetErrorString(e)));
 
   int rc = batch_grid(st, &p->grid);
   if (rc) return bail(rc);
-  p->grid = std::min<uint32_t>(p->grid, p->nitems);
+  p->grid = std::max<uint32_t>(1, std::min<uint32_t>(p->grid, p->nitems));
 
   if (graph) {
     hipStream_t cap;
@@ -258,7 +326,7 @@ static int build_plan(const zoda_square* squares, uint32_t count, bool graph, zo
 
 extern "C" int zoda_hip_batch_plan_create(const zoda_square* squares, uint32_t count,
                                           zoda_batch_plan** plan) {
-  return build_plan(squares, count, true, plan);
+  return build_plan(squares, count, true, true, plan);
 }
 
 extern "C" int zoda_hip_batch_plan_launch(zoda_batch_plan* plan, void* stream) {
@@ -278,7 +346,7 @@ extern "C" int zoda_hip_batch_plan_destroy(zoda_batch_plan* plan) {
 
 extern "C" int zoda_hip_encode_batch(const zoda_square* squares, uint32_t count, void* stream) {
   zoda_batch_plan* p = nullptr;
-  ZODA_TRY(build_plan(squares, count, false, &p));
+  ZODA_TRY(build_plan(squares, count, false, true, &p));
   int rc = launch_batch_direct(p, (hipStream_t)stream);
   hipError_t e = hipStreamSynchronize((hipStream_t)stream);
   free_plan(p);
@@ -345,7 +413,7 @@ extern "C" int zoda_hip_autotune(const uint32_t* sides, uint32_t count, int iter
       st->batch_variant[L] = v;
       zoda_square sq{din, dout, side, (uint32_t)cols, side, 0};
       zoda_batch_plan* p = nullptr;
-      rc = build_plan(&sq, 1, false, &p);
+      rc = build_plan(&sq, 1, false, false, &p);
       if (rc) break;
       float vbest = 1e30f;
       for (int it = 0; it <= iters && !rc; it++) {

@@ -97,9 +97,16 @@ inline uint32_t encode_table_key(int L, int r1, int r2, int rd) {
 // Constant tables of the fused encode for side 2^L and digit split (r1, r2, rd).
 int get_encode_tables(DeviceState* st, int L, int r1, int r2, int rd, const EncodeTableSet** out);
 
-// Fused column encode on device memory (encode.hip).
+// Fused column encode on device memory (encode.hip).  With dry, only the
+// one-time set-up (tables, kernel attributes) is done and nothing launches --
+// the set-up calls are not allowed inside a stream capture.
 int launch_encode(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols,
-                  uint32_t n, uint32_t* d_out, int layout, hipStream_t stream);
+                  uint32_t n, uint32_t* d_out, int layout, hipStream_t stream, bool dry = false);
+
+// Squares of side >= 2^DIRECT_MIN_LOG fill the device with their own fused
+// kernel: batched and served work runs them as separate launches next to the
+// persistent work queue of the smaller squares.
+constexpr uint32_t DIRECT_MIN_LOG = 8;
 
 // Seeded uniform generator (api.hip).
 int launch_fill_uniform(uint32_t* d_out, size_t count, uint64_t seed, hipStream_t stream);

@@ -147,7 +147,7 @@ static int kernel_setup(DeviceState* st, const void* fn, int nt, size_t lds_byte
 
 template <class PL>
 static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols,
-                       uint32_t* d_out, int layout, hipStream_t stream) {
+                       uint32_t* d_out, int layout, hipStream_t stream, bool dry) {
   const EncodeTableSet* set = nullptr;
   ZODA_TRY(get_encode_tables(st, PL::L, PL::R1, PL::R2, PL::RD, &set));
   const EncodeTabs& T = *set->tabs;
@@ -161,6 +161,7 @@ static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32
       const void* pfn = (const void*)encode_pipelined<PL>;
       uint32_t grid = 0;
       ZODA_TRY(kernel_setup(st, pfn, PL::NT, lds_bytes, true, &grid));
+      if (dry) return 0;
       uint32_t ntiles = cols / PL::W;
       if (grid > ntiles) grid = ntiles;
       void* pargs[] = {(void*)&d_in, (void*)&d_out, (void*)&cols, (void*)&ntiles, (void*)&T};
@@ -172,6 +173,7 @@ static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32
                         : (row ? (const void*)encode_kernel<PL, false, 0> : (const void*)encode_kernel<PL, false, 1>);
   uint32_t unused = 0;
   ZODA_TRY(kernel_setup(st, fn, PL::NT, lds_bytes, false, &unused));
+  if (dry) return 0;
   const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
   void* args[] = {(void*)&d_in, (void*)&d_out, (void*)&k, (void*)&cols, (void*)&T};
   ZODA_HIP(hipLaunchKernel(fn, dim3(ntiles), dim3(PL::NT), args, lds_bytes, stream));
@@ -230,14 +232,15 @@ static int encode_general(DeviceState* st, const uint32_t* d_in, uint32_t k, uin
 }
 
 int launch_encode(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols, uint32_t n,
-                  uint32_t* d_out, int layout, hipStream_t stream) {
+                  uint32_t* d_out, int layout, hipStream_t stream, bool dry) {
   const uint32_t log_nk = log2_ceil(k);
   const uint32_t log_nkn = log2_ceil((uint64_t)k + n);
   if (log_nkn == log_nk + 1 && (int)log_nk <= ENCODE_MAX_LOG) {
     return with_encode_plan((int)log_nk, [&](auto plan) -> int {
-      return launch_plan<decltype(plan)>(st, d_in, k, cols, d_out, layout, stream);
+      return launch_plan<decltype(plan)>(st, d_in, k, cols, d_out, layout, stream, dry);
     });
   }
+  if (dry) return 0;
   return encode_general(st, d_in, k, cols, log_nk, log_nkn, d_out, layout, stream);
 }
 

@@ -52,39 +52,50 @@ int with_encode_plan(int L, F&& f) {
 namespace zoda {
 
 // Plans of the batched persistent kernel: one block size (256) for every
-// side, LDS <= 64 KiB so two work-groups share a CU, and two variants per
-// side (tile width W, digit split = radix and stages per register pass) for
-// the autotuner to choose from.
+// side, at most 80 KiB of LDS so two work-groups share a CU, and two variants
+// per side (tile width W, digit split = radix and stages per register pass)
+// for the autotuner to choose from.  Up to side 1024 the twiddle tables sit
+// in LDS behind the tile (loaded when a work-group changes plan); the two
+// largest sides read them from global memory.
 constexpr int BATCH_NT = 256;
 constexpr int BATCH_MIN_LOG = 0;
 constexpr int BATCH_MAX_LOG = 12;
 constexpr int BATCH_VARIANTS = 2;
-constexpr int BATCH_LDS_WORDS = 16384;
+// 80 KiB less 64 bytes for the kernel's static LDS (its work-item slot).
+constexpr int BATCH_LDS_WORDS = 20480 - 16;
 
-//                          L   W    NT        R1 R2 RD
-using B0a = EncodePlan<0, 256, BATCH_NT, 0, 0, 0>;
-using B1a = EncodePlan<1, 256, BATCH_NT, 0, 0, 1>;
-using B2a = EncodePlan<2, 256, BATCH_NT, 0, 0, 2>;
-using B3a = EncodePlan<3, 256, BATCH_NT, 0, 0, 3>;
-using B4a = EncodePlan<4, 256, BATCH_NT, 0, 0, 4>;
-using B4b = EncodePlan<4, 64, BATCH_NT, 2, 0, 2>;
-using B5a = EncodePlan<5, 256, BATCH_NT, 0, 0, 5>;
-using B5b = EncodePlan<5, 64, BATCH_NT, 3, 0, 2>;
-using B6a = EncodePlan<6, 64, BATCH_NT, 3, 0, 3>;
-using B6b = EncodePlan<6, 32, BATCH_NT, 3, 0, 3>;
-using B7a = EncodePlan<7, 64, BATCH_NT, 4, 0, 3>;
-using B7b = EncodePlan<7, 32, BATCH_NT, 4, 0, 3>;
-using B8a = EncodePlan<8, 64, BATCH_NT, 4, 0, 4>;
-using B8b = EncodePlan<8, 32, BATCH_NT, 3, 2, 3>;
-using B9a = EncodePlan<9, 32, BATCH_NT, 3, 3, 3>;
-using B9b = EncodePlan<9, 32, BATCH_NT, 5, 0, 4>;
-using B10a = EncodePlan<10, 16, BATCH_NT, 4, 3, 3>;
-using B10b = EncodePlan<10, 16, BATCH_NT, 5, 0, 5>;
+//                          L   W    NT        R1 R2 RD  LDS twiddles
+using B0a = EncodePlan<0, 256, BATCH_NT, 0, 0, 0, true>;
+using B1a = EncodePlan<1, 256, BATCH_NT, 0, 0, 1, true>;
+using B2a = EncodePlan<2, 256, BATCH_NT, 0, 0, 2, true>;
+using B3a = EncodePlan<3, 256, BATCH_NT, 0, 0, 3, true>;
+using B4a = EncodePlan<4, 256, BATCH_NT, 0, 0, 4, true>;
+using B4b = EncodePlan<4, 64, BATCH_NT, 2, 0, 2, true>;
+using B5a = EncodePlan<5, 256, BATCH_NT, 0, 0, 5, true>;
+using B5b = EncodePlan<5, 64, BATCH_NT, 3, 0, 2, true>;
+using B6a = EncodePlan<6, 64, BATCH_NT, 3, 0, 3, true>;
+using B6b = EncodePlan<6, 32, BATCH_NT, 3, 0, 3, true>;
+using B7a = EncodePlan<7, 64, BATCH_NT, 4, 0, 3, true>;
+using B7b = EncodePlan<7, 32, BATCH_NT, 4, 0, 3, true>;
+using B8a = EncodePlan<8, 64, BATCH_NT, 4, 0, 4, true>;
+using B8b = EncodePlan<8, 32, BATCH_NT, 3, 2, 3, true>;
+using B9a = EncodePlan<9, 32, BATCH_NT, 3, 3, 3, true>;
+using B9b = EncodePlan<9, 32, BATCH_NT, 5, 0, 4, true>;
+using B10a = EncodePlan<10, 16, BATCH_NT, 4, 3, 3, true>;
+using B10b = EncodePlan<10, 16, BATCH_NT, 5, 0, 5, true>;
 using B11a = EncodePlan<11, 8, BATCH_NT, 4, 4, 3>;
 using B12a = EncodePlan<12, 4, BATCH_NT, 4, 4, 4>;
 
 // Calls f(Plan{}) for batched plan (L, variant); variants a side lacks fall
 // back to variant 0.  Returns -1 for L out of range.
+template <class PL>
+constexpr bool fits_batch() {
+  return PL::NT == BATCH_NT && PL::LDS_WORDS <= BATCH_LDS_WORDS;
+}
+static_assert(fits_batch<B8a>() && fits_batch<B9a>() && fits_batch<B9b>() && fits_batch<B10a>() &&
+                  fits_batch<B10b>() && fits_batch<B11a>() && fits_batch<B12a>(),
+              "batched plans must fit the batched kernel's LDS");
+
 template <class F>
 ZHD int with_batch_plan(int L, int variant, F&& f) {
   switch (L * BATCH_VARIANTS + (variant & 1)) {

@@ -3,8 +3,8 @@
 // Requests (device-resident squares, as zoda_square) queue in arrival order.
 // Each of `slots` launch slots owns a stream, a completion event and a work
 // counter.  Whenever a slot is free it takes the next launch:
-//   * a large request (side >= 2^SERVE_BIG_LOG: its own fused kernel fills the
-//     device) at the head of the queue goes alone through the single-square
+//   * a large request (side >= 2^DIRECT_MIN_LOG: its own fused kernel fills
+//     the device) at the head of the queue goes alone through the single-square
 //     kernel; at most slots - 1 of them run at once, in arrival order;
 //   * otherwise every queued small request that fits the HBM budget next to
 //     the requests on the GPU goes out as one persistent batched launch whose
@@ -30,7 +30,6 @@ namespace zoda {
 
 namespace {
 
-constexpr uint32_t SERVE_BIG_LOG = 8;
 
 double now_us() {
   using namespace std::chrono;
@@ -105,7 +104,7 @@ class Server {
     r.sq = sq;
     r.arrival_us = arrival_us;
     r.bytes = request_bytes(sq);
-    r.big = log2_ceil(sq.k) >= SERVE_BIG_LOG;
+    r.big = log2_ceil(sq.k) >= DIRECT_MIN_LOG;
     if (!r.big) {  // resolve the batched plan and its tables now, off the launch path
       SquareDesc d;
       ZODA_TRY(describe_square(st_, sq, &d, tabs_.data(), have_.get()));
