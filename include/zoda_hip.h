@@ -118,10 +118,12 @@ typedef struct zoda_square {
   uint32_t reserved; /* 0 */
 } zoda_square;
 
-/* Encodes `count` squares of any mix of sides in one persistent launch that
- * drains a device work queue of (square, column tile) items, largest side
- * first.  Synchronous on `stream`.  Results equal zoda_hip_encode_device of
- * each square alone. */
+/* Encodes `count` squares of any mix of sides: squares of side < 256 in one
+ * persistent launch that drains a device work queue of (square, column tile)
+ * items, largest side first; squares of side >= 256 (whose own fused kernel
+ * fills the device) as their own launches on a parallel branch.  Synchronous
+ * on `stream`.  Results equal zoda_hip_encode_device of each square alone.
+ * No counterpart in the reference (a batching pattern over its encode). */
 int zoda_hip_encode_batch(const zoda_square* squares, uint32_t count, void* stream);
 
 /* A batch captured once into a hipGraph ([queue reset, persistent kernel])
@@ -154,11 +156,15 @@ typedef struct zoda_serve_stats {
 
 /* Continuous-batching encode server on the current device.  Requests are
  * device-resident squares (zoda_square).  Each of `slots` launch slots
- * (1..16) owns a stream; whenever one is free, every queued request whose
- * in+out bytes fit `hbm_budget` next to the requests still on the GPU goes out
- * as one persistent batched launch, overlapping the launches in flight.  A
- * scheduler thread runs until zoda_server_destroy.  A request larger than the
- * whole budget is rejected with ZODA_EBUDGET (its ticket reads state 3). */
+ * (1..16) owns a stream.  Whenever one is free it takes the next launch: a
+ * large request (side >= 256) at the head of the queue alone through the
+ * fused single-square kernel (at most slots - 1 at once, in arrival order),
+ * or else every queued small request that fits `hbm_budget` next to the
+ * requests on the GPU as one persistent batched launch, overlapping the
+ * launches in flight.  A scheduler thread runs until zoda_server_destroy.  A
+ * request larger than the whole budget is rejected with ZODA_EBUDGET (its
+ * ticket reads state 3).  No counterpart in the reference (a serving pattern
+ * over its encode). */
 typedef struct zoda_server zoda_server;
 int zoda_server_create(uint64_t hbm_budget, uint32_t slots, zoda_server** server);
 int zoda_server_submit(zoda_server* server, const zoda_square* request, uint64_t* ticket);
