@@ -41,13 +41,14 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the 96-square batched comparison")
     ap.add_argument("--no-served", action="store_true", help="skip the served-trace comparison")
     ap.add_argument("--served-requests", type=int, default=1000)
+    ap.add_argument("--no-rlc", action="store_true", help="skip the RLC leg")
     ap.add_argument("--only-encode", action="store_true",
                     help="time the encode only (no end-to-end, batched, served or CPU legs); for profiling")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="instead of benchmarking, run 3 device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.only_encode:
-        args.no_e2e = args.no_cpu = args.no_batched = args.no_served = True
+        args.no_e2e = args.no_cpu = args.no_batched = args.no_served = args.no_rlc = True
     return args
 
 
@@ -213,6 +214,40 @@ def served_vs_plain(zoda_hip, torch, seed: int, count: int, rates=(2000, 8000)):
     }
 
 
+def rlc_leg(zoda_hip, torch, t_out, reps: int = 10):
+    """RLC (src/zoda/rlc.rs:32-40) of every row of the encoded square with the
+    column coefficients of a data root (derived on the device, rlc.rs:48-66):
+    one streaming pass over the 2s x C output, HIP events, p50 of `reps`."""
+    rows, cols = t_out.shape
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    coeffs = zoda_hip.rlc_coefficients("0" * 64, cols)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    coeff_ms = e0.elapsed_time(e1)
+    zoda_hip.rlc_rows(t_out, coeffs)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        zoda_hip.rlc_rows(t_out, coeffs)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms.append(a.elapsed_time(b))
+    ms.sort()
+    t = ms[len(ms) // 2]
+    alg = rows * cols * 4 + cols * 16
+    return {
+        "workload": f"RLC of all {rows} rows of the {rows}x{cols} encoded square, 4 limbs",
+        "ms_p50": round(t, 4),
+        "roofline": {"bound": "hbm", "achieved": round(alg / (t / 1e3) / 1e9, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(alg / (t / 1e3) / 1e9 / PEAK_HBM_GBS, 4)},
+        "coefficients_ms": round(coeff_ms, 3),
+    }
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -348,6 +383,9 @@ def main():
         except Exception as ex:  # report, do not fail the device-resident line
             e2e = {"error": str(ex)}
         result["end_to_end_pinned"] = e2e
+
+    if not args.no_rlc:
+        result["rlc"] = rlc_leg(zoda_hip, torch, t_out)
 
     del t_in, t_out
     torch.cuda.empty_cache()

@@ -183,6 +183,33 @@ int zoda_hip_serve_trace(const zoda_square* requests, const double* arrival_us, 
                          uint64_t hbm_budget, uint32_t slots, int mode, double* latency_us,
                          zoda_serve_stats* stats);
 
+/* ---- Random linear combinations of encoded rows (commitment checks) ----
+ *
+ * Replaces the CPU loops of src/zoda/rlc.rs:32-66 and the data root of
+ * src/benchmarks/utils.rs:51-59, as used by the validation of utils.rs:221-272
+ * and the encoder check of src/zoda/babybear.rs:164-192. */
+
+/* Coefficients of columns 0..cols-1 derived from the data root text
+ * (generate_deterministic_coefficients, rlc.rs:48-66): SHA-256 of the text
+ * followed by the column index as 8 little-endian bytes; limb l is the
+ * big-endian u64 in digest bytes [8l, 8l+8) mod p.  d_coeffs[c*4 + l].
+ * Synchronous on `stream`. */
+int zoda_hip_rlc_coefficients(const char* data_root, uint32_t root_len, uint32_t cols, uint32_t* d_coeffs,
+                              void* stream);
+
+/* rlc_row (rlc.rs:32-40) of `count` rows of the row-major rows x cols device
+ * matrix d_square: row d_row_index[i] (device array; NULL means rows
+ * 0..count-1, and then count <= rows), d_out[i*4 + l] = sum over columns of
+ * row[c] * d_coeffs[c*4 + l] mod p.  A row index >= rows gives zeros.
+ * Asynchronous on `stream`. */
+int zoda_hip_rlc_rows(const uint32_t* d_square, uint32_t rows, uint32_t cols, const uint32_t* d_row_index,
+                      uint32_t count, const uint32_t* d_coeffs, uint32_t* d_out, void* stream);
+
+/* compute_data_root (utils.rs:51-59): SHA-256 of `count` host u32 values as
+ * little-endian bytes in order, written as 64 lowercase hex digits and a NUL
+ * to hex_out (65 bytes).  Runs on the host (one sequential hash chain). */
+int zoda_data_root(const uint32_t* values, size_t count, char* hex_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,10 @@ def sim_lib():
         lib.zoda_sim_encode_variant.restype = ctypes.c_int
         lib.zoda_sim_encode_variant.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                                 u32p, ctypes.c_int, ctypes.c_int]
+        lib.zoda_sim_sha256.restype = None
+        lib.zoda_sim_sha256.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_void_p]
+        lib.zoda_sim_rlc_coefficient.restype = None
+        lib.zoda_sim_rlc_coefficient.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p]
         lib.zoda_sim_encode_persistent.restype = ctypes.c_int
         lib.zoda_sim_encode_persistent.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p]
         _sim = lib

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../zoda-cu_amd/csrc/encode_plans.hpp"
+#include "../../zoda-cu_amd/csrc/sha256.hpp"
 #include "../../zoda-cu_amd/csrc/encode_tables.hpp"
 
 using namespace zoda;
@@ -117,4 +118,16 @@ extern "C" int zoda_sim_encode_variant(const uint32_t* in, uint32_t k, uint32_t 
 extern "C" int zoda_sim_encode(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t n,
                                uint32_t* out, int layout) {
   return zoda_sim_encode_variant(in, k, cols, n, out, layout, -1);
+}
+
+// Host runs of the device code's SHA-256 and RLC coefficient derivation.
+extern "C" void zoda_sim_sha256(const uint8_t* data, size_t len, uint8_t* out) {
+  Sha256 h;
+  h.init();
+  h.update(data, len);
+  h.final(out);
+}
+
+extern "C" void zoda_sim_rlc_coefficient(const uint8_t* root, uint32_t len, uint64_t col, uint32_t* out) {
+  rlc_coefficient(root, len, col, out);
 }

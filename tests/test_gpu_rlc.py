@@ -1,0 +1,81 @@
+"""GPU parity of the random linear combinations (src/zoda/rlc.rs:32-66) and
+the commitment check they serve (src/benchmarks/utils.rs:221-272): the
+device-derived coefficients equal the oracle's SHA-256 derivation, the device
+row RLC equals the oracle's rlc_row, and on the 2048^2 square the RLC of every
+extended row equals the extension of the original rows' RLC (the two are
+F_p-linear and commute)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+P = O.P
+
+
+@pytest.fixture(scope="module")
+def zh():
+    import zoda_hip
+    assert zoda_hip.gpu_available(), "no HIP device"
+    return zoda_hip
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    return torch
+
+
+def u32(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+@pytest.mark.parametrize("root", ["", "c0ffee", "a" * 64, "3f" * 40])
+def test_coefficients_match_oracle(zh, torch, root):
+    print(f"[case] rlc coefficients root_len={len(root)}", flush=True)
+    cols = 3000
+    got = u32(zh.rlc_coefficients(root, cols))
+    assert np.array_equal(got, O.generate_deterministic_coefficients(root, cols))
+
+
+@pytest.mark.parametrize("rows,cols", [(1, 1), (37, 5000), (16, 4096), (17, 4097), (5, 70000)])
+def test_rlc_rows_match_oracle(zh, torch, rows, cols):
+    print(f"[case] rlc rows {rows}x{cols}", flush=True)
+    rng = np.random.default_rng(rows * 100003 + cols)
+    sq = rng.integers(0, P, size=(rows, cols), dtype=np.uint32)
+    # extreme values in the first row: every product near (p-1)^2
+    sq[0, :] = P - 1
+    coeffs = zh.rlc_coefficients("f" * 64, cols)
+    coeffs[: min(cols, 3)] = P - 1
+    t_sq = torch.from_numpy(sq.view(np.int32)).cuda()
+    got = u32(zh.rlc_rows(t_sq, coeffs))
+    c = u32(coeffs)
+    for r in range(rows):
+        assert np.array_equal(got[r], O.rlc_row(sq[r], c)), r
+    # a row subset, repeated rows and an index past the last row (zeros)
+    idx = np.array([rows - 1, 0, 0, rows + 5], dtype=np.int32)
+    got = u32(zh.rlc_rows(t_sq, coeffs, torch.from_numpy(idx).cuda()))
+    assert np.array_equal(got[0], O.rlc_row(sq[rows - 1], c))
+    assert np.array_equal(got[1], O.rlc_row(sq[0], c)) and np.array_equal(got[2], got[1])
+    assert not got[3].any()
+
+
+def test_commutes_with_the_extension_2048(zh, torch):
+    # RLC(extended row i) = sum_j G[i][j] RLC(original row j) for every one of
+    # the 4096 rows of the 2048^2 square: each limb of the original rows' RLC,
+    # extended by the oracle (INTT-2048, pad, NTT-4096), must equal the RLC of
+    # the device-encoded rows.
+    side = 2048
+    cols = 128 * side
+    print(f"[case] rlc commute {side}x{cols}", flush=True)
+    t_in = torch.empty((side, cols), dtype=torch.int32, device="cuda")
+    zh.fill_uniform(t_in, seed=2048)
+    t_out = zh.encode_tensor(t_in)
+    coeffs = zh.rlc_coefficients("ab" * 32, cols)
+    ext = u32(zh.rlc_rows(t_out, coeffs))          # (4096, 4)
+    orig = u32(zh.rlc_rows(t_in, coeffs))          # (2048, 4)
+    for limb in range(4):
+        expect = O.encode(orig[:, limb].reshape(side, 1), threads=4).ravel()
+        assert np.array_equal(ext[:, limb], expect), limb
+    # and the systematic rows: the even extended rows are the original rows
+    assert np.array_equal(ext[0::2], orig)

@@ -70,6 +70,10 @@ def _load():
                                                 _vp, ctypes.c_int]),
         "zoda_hip_fill_uniform": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_uint64, _vp]),
         "zoda_hip_encode_batch": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp]),
+        "zoda_hip_rlc_coefficients": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp]),
+        "zoda_hip_rlc_rows": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp,
+                                             _vp]),
+        "zoda_data_root": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_char_p]),
         "zoda_hip_batch_plan_create": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(_vp)]),
         "zoda_hip_batch_plan_launch": (ctypes.c_int, [_vp, _vp]),
         "zoda_hip_batch_plan_destroy": (ctypes.c_int, [_vp]),
@@ -426,3 +430,42 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ---- random linear combinations (src/zoda/rlc.rs) ----
+
+def rlc_coefficients(data_root: str, cols: int, out=None):
+    """Per-column RLC coefficients from the data root text (rlc.rs:48-66) as a
+    (cols, 4) int32 torch tensor on the current device (u32 bit patterns)."""
+    import torch
+    if out is None:
+        out = torch.empty((cols, 4), dtype=torch.int32, device="cuda")
+    root = data_root.encode()
+    _check(lib.zoda_hip_rlc_coefficients(root, len(root), cols, out.data_ptr(),
+                                         torch.cuda.current_stream().cuda_stream),
+           "zoda_hip_rlc_coefficients")
+    return out
+
+
+def rlc_rows(square, coeffs, rows=None):
+    """rlc_row (rlc.rs:32-40) of rows of a row-major (R, C) int32 device tensor
+    (all rows, or the row indices given as a device int32 tensor): an
+    (n, 4) int32 tensor of u32 bit patterns."""
+    import torch
+    assert square.is_contiguous() and square.dim() == 2
+    R, C = square.shape
+    assert coeffs.shape == (C, 4) and coeffs.is_contiguous()
+    count = R if rows is None else rows.numel()
+    out = torch.empty((count, 4), dtype=torch.int32, device=square.device)
+    _check(lib.zoda_hip_rlc_rows(square.data_ptr(), R, C, None if rows is None else rows.data_ptr(), count,
+                                 coeffs.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream),
+           "zoda_hip_rlc_rows")
+    return out
+
+
+def data_root(values) -> str:
+    """compute_data_root (utils.rs:51-59) of host u32 values in row order."""
+    a = np.ascontiguousarray(values, dtype=np.uint32)
+    buf = ctypes.create_string_buffer(65)
+    _check(lib.zoda_data_root(a.ctypes.data, a.size, buf), "zoda_data_root")
+    return buf.value.decode()
