@@ -42,13 +42,14 @@ def parse():
     ap.add_argument("--no-served", action="store_true", help="skip the served-trace comparison")
     ap.add_argument("--served-requests", type=int, default=1000)
     ap.add_argument("--no-rlc", action="store_true", help="skip the RLC leg")
+    ap.add_argument("--no-decode", action="store_true", help="skip the erasure-decode leg")
     ap.add_argument("--only-encode", action="store_true",
                     help="time the encode only (no end-to-end, batched, served or CPU legs); for profiling")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="instead of benchmarking, run 3 device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.only_encode:
-        args.no_e2e = args.no_cpu = args.no_batched = args.no_served = args.no_rlc = True
+        args.no_e2e = args.no_cpu = args.no_batched = args.no_served = args.no_rlc = args.no_decode = True
     return args
 
 
@@ -248,6 +249,32 @@ def rlc_leg(zoda_hip, torch, t_out, reps: int = 10):
     }
 
 
+def decode_leg(zoda_hip, torch, t_in, t_out, seed: int, reps: int = 3):
+    """Erasure decode of the square from a random half of its 2s rows (the
+    data back, bit-exact), wall clock around the synchronous call (it includes
+    the O(s^2) host set-up of the erasure pattern), median of `reps`."""
+    import numpy as np
+    side, cols = t_in.shape
+    known = np.random.default_rng(seed).choice(2 * side, size=side, replace=False)
+    out = torch.empty_like(t_in)
+    zoda_hip.decode_tensor(t_out, known, out=out)
+    torch.cuda.synchronize()
+    ok = bool(torch.equal(out, t_in))
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        zoda_hip.decode_tensor(t_out, known, out=out)
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    t = times[len(times) // 2]
+    return {
+        "workload": f"decode the {side}x{cols} data from {side} random rows of {2 * side}",
+        "ms_p50": round(t * 1e3, 2),
+        "payload_gbps": round(side * side * SHARE_BYTES * 8 / 1e9 / t, 1),
+        "bit_exact": ok,
+    }
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -386,6 +413,8 @@ def main():
 
     if not args.no_rlc:
         result["rlc"] = rlc_leg(zoda_hip, torch, t_out)
+    if not args.no_decode:
+        result["decode"] = decode_leg(zoda_hip, torch, t_in, t_out, args.seed)
 
     del t_in, t_out
     torch.cuda.empty_cache()

@@ -210,6 +210,18 @@ int zoda_hip_rlc_rows(const uint32_t* d_square, uint32_t rows, uint32_t cols, co
  * to hex_out (65 bytes).  Runs on the host (one sequential hash chain). */
 int zoda_data_root(const uint32_t* values, size_t count, char* hex_out);
 
+/* ---- Erasure decoding ----
+ *
+ * The data square back from any k of the 2k rows of its extended square
+ * (device memory, row-major).  No counterpart in the reference, which only
+ * checks that every column is a Reed-Solomon codeword with the data at the
+ * even rows (src/zoda/babybear.rs:195-211); this inverts that map.
+ * d_enc: 2k x cols, rows not listed are never read.  known_rows: host array of
+ * row indices (the first k distinct ones are used).  d_data: k x cols.  k a
+ * power of two <= 2^15.  Synchronous on `stream`. */
+int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_t cols, const uint32_t* known_rows,
+                           uint32_t nknown, uint32_t* d_data, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

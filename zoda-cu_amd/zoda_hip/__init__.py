@@ -74,6 +74,8 @@ def _load():
         "zoda_hip_rlc_rows": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp,
                                              _vp]),
         "zoda_data_root": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_char_p]),
+        "zoda_hip_decode_device": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp,
+                                                  _vp]),
         "zoda_hip_batch_plan_create": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(_vp)]),
         "zoda_hip_batch_plan_launch": (ctypes.c_int, [_vp, _vp]),
         "zoda_hip_batch_plan_destroy": (ctypes.c_int, [_vp]),
@@ -469,3 +471,20 @@ def data_root(values) -> str:
     buf = ctypes.create_string_buffer(65)
     _check(lib.zoda_data_root(a.ctypes.data, a.size, buf), "zoda_data_root")
     return buf.value.decode()
+
+
+def decode_tensor(encoded, known_rows, out=None):
+    """The k x cols data square back from the 2k x cols extended square
+    (int32 device tensor, row-major), using the first k distinct rows listed
+    in known_rows; the other rows are never read."""
+    import torch
+    assert encoded.is_contiguous() and encoded.dim() == 2
+    n, cols = encoded.shape
+    k = n // 2
+    if out is None:
+        out = torch.empty((k, cols), dtype=torch.int32, device=encoded.device)
+    rows = np.ascontiguousarray(known_rows, dtype=np.uint32)
+    _check(lib.zoda_hip_decode_device(encoded.data_ptr(), k, cols, rows.ctypes.data, rows.size, out.data_ptr(),
+                                      torch.cuda.current_stream().cuda_stream),
+           "zoda_hip_decode_device")
+    return out
