@@ -215,6 +215,29 @@ def served_vs_plain(zoda_hip, torch, seed: int, count: int, rates=(2000, 8000)):
     }
 
 
+def measured_copy_gbps(torch, like, reps: int = 5):
+    """Device-to-device copy bandwidth on a buffer the size of the input (read +
+    write bytes over the HIP-event time, p50): the practical ceiling the
+    roofline is also reported against."""
+    src = torch.empty_like(like)
+    dst = torch.empty_like(like)
+    stream = torch.cuda.current_stream()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        dst.copy_(src)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms.append(a.elapsed_time(b))
+    ms.sort()
+    nbytes = 2 * src.numel() * src.element_size()
+    del src, dst
+    return round(nbytes / (ms[len(ms) // 2] / 1e3) / 1e9, 1)
+
+
 def rlc_leg(zoda_hip, torch, t_out, reps: int = 10):
     """RLC (src/zoda/rlc.rs:32-40) of every row of the encoded square with the
     column coefficients of a data root (derived on the device, rlc.rs:48-66):
@@ -354,6 +377,7 @@ def main():
     value = payload * 8 / 1e9 / (ms_per_step / 1e3)
     achieved = alg_bytes / (avg_ms / 1e3) / 1e9
     traffic = traffic_from_profile(side)
+    copy_gbps = measured_copy_gbps(torch, t_in)
 
     result = {
         "metric": METRIC,
@@ -383,6 +407,8 @@ def main():
             "algorithmic_bytes": alg_bytes,
             "kernel_ms_avg": round(avg_ms, 4),
             "kernel_ms_p50": round(p50_ms, 4),
+            "copy_gbps": copy_gbps,
+            "frac_of_copy": round(achieved / copy_gbps, 4) if copy_gbps else None,
         },
     }
 

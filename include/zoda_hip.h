@@ -200,7 +200,8 @@ int zoda_hip_rlc_coefficients(const char* data_root, uint32_t root_len, uint32_t
 /* rlc_row (rlc.rs:32-40) of `count` rows of the row-major rows x cols device
  * matrix d_square: row d_row_index[i] (device array; NULL means rows
  * 0..count-1, and then count <= rows), d_out[i*4 + l] = sum over columns of
- * row[c] * d_coeffs[c*4 + l] mod p.  A row index >= rows gives zeros.
+ * row[c] * d_coeffs[c*4 + l] mod p (d_coeffs 16-byte aligned).  A row index
+ * >= rows gives zeros.
  * Asynchronous on `stream`. */
 int zoda_hip_rlc_rows(const uint32_t* d_square, uint32_t rows, uint32_t cols, const uint32_t* d_row_index,
                       uint32_t count, const uint32_t* d_coeffs, uint32_t* d_out, void* stream);

@@ -5,11 +5,13 @@
 //   rlc(row)[l] = sum over columns c of row[c] * coeff[c][l]  (mod p), l < 4
 //
 // The row kernel streams the row-major square once.  A work-group owns a
-// block of RLC_ROWS rows by RLC_COLS columns: each thread keeps the
-// coefficients of its RLC_PER columns in registers for all the rows, and
-// accumulates the 31 x 31-bit products in 64-bit integers, folding the high
-// word back with 2^32 = 2^32 - 2p (mod p) every three products so the sum
-// never overflows.  A second kernel adds the column blocks' partial sums.
+// block of RLC_COLS columns, keeps their coefficients in LDS, and each of its
+// waves takes whole rows of the block: lane l reads columns l, l + 64, ...
+// (every load instruction covers 256 contiguous bytes), accumulates the
+// 31 x 31-bit products in 64-bit integers -- folding the high word back with
+// 2^32 = 2^32 - 2p (mod p) every three products so the sum never overflows --
+// and the wave adds its lanes once per row.  A second kernel adds the column
+// blocks' partial sums.
 #include <string.h>
 
 #include <vector>
@@ -22,9 +24,9 @@ namespace zoda {
 namespace {
 
 constexpr int RLC_NT = 256;
-constexpr int RLC_PER = 16;                      // columns per thread
-constexpr uint32_t RLC_COLS = RLC_NT * RLC_PER;  // columns per work-group
-constexpr uint32_t RLC_ROWS = 16;                // rows per work-group
+constexpr uint32_t RLC_COLS = 2048;  // columns per work-group (32 KiB of coefficients)
+constexpr uint32_t RLC_ROWS = 32;    // rows per work-group
+constexpr int RLC_PER = RLC_COLS / 64;
 constexpr uint64_t TWO32_MOD_P = 4294967296ull - 2ull * 2013265921ull;
 
 // x < 2^64 -> x' = x (mod p) with x' < 2^61.
@@ -51,54 +53,64 @@ __global__ void rlc_coeff_kernel(const uint8_t* __restrict__ root, uint32_t root
 // or i without an index); a requested row outside the square contributes 0.
 __global__ __launch_bounds__(RLC_NT) void rlc_rows_kernel(const uint32_t* __restrict__ sq, uint32_t sq_rows,
                                                           uint32_t cols, const uint32_t* __restrict__ rowidx,
-                                                          uint32_t rows, const uint32_t* __restrict__ coeffs,
+                                                          uint32_t rows, const uint4* __restrict__ coeffs,
                                                           uint32_t* __restrict__ partial) {
-  __shared__ uint64_t red[RLC_NT / 64][4];
+  __shared__ uint4 cs[RLC_COLS];
   const uint32_t blk = blockIdx.x, nblk = gridDim.x;
-  const uint32_t r0 = blockIdx.y * RLC_ROWS;
+  const uint32_t c0 = blk * RLC_COLS, r0 = blockIdx.y * RLC_ROWS;
   const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  uint32_t cf[RLC_PER][4];
-  bool live[RLC_PER];
-#pragma unroll
-  for (int j = 0; j < RLC_PER; j++) {
-    const uint32_t c = blk * RLC_COLS + (uint32_t)j * RLC_NT + (uint32_t)tid;
-    live[j] = c < cols;
-#pragma unroll
-    for (int l = 0; l < 4; l++) cf[j][l] = live[j] ? coeffs[(size_t)c * 4 + l] : 0u;
-  }
-  for (uint32_t i = 0; i < RLC_ROWS && r0 + i < rows; i++) {
+  for (uint32_t i = (uint32_t)tid; i < RLC_COLS; i += RLC_NT)
+    cs[i] = c0 + i < cols ? coeffs[c0 + i] : make_uint4(0u, 0u, 0u, 0u);
+  __syncthreads();
+  const bool full = c0 + RLC_COLS <= cols;
+  for (uint32_t i = (uint32_t)wave; i < RLC_ROWS && r0 + i < rows; i += RLC_NT / 64) {
     const uint32_t r = rowidx ? rowidx[r0 + i] : r0 + i;
-    const bool in_sq = r < sq_rows;
-    const uint32_t* src = sq + (size_t)(in_sq ? r : 0) * cols + blk * RLC_COLS + tid;
-    uint32_t v[RLC_PER];
+    // Two accumulator sets (even and odd j) halve the dependency chains.
+    uint64_t acc[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    if (r < sq_rows) {
+      const uint32_t* src = sq + (size_t)r * cols + c0 + lane;
 #pragma unroll
-    for (int j = 0; j < RLC_PER; j++) v[j] = (live[j] && in_sq) ? src[(size_t)j * RLC_NT] : 0u;
-    uint64_t acc[4] = {0, 0, 0, 0};
+      for (int jb = 0; jb < RLC_PER; jb += 16) {
+        uint32_t v[16];
 #pragma unroll
-    for (int j = 0; j < RLC_PER; j++) {
+        for (int j = 0; j < 16; j++) {
+          const uint32_t c = (uint32_t)(jb + j) * 64 + (uint32_t)lane;
+          v[j] = (full || c0 + c < cols) ? src[(size_t)(jb + j) * 64] : 0u;
+        }
 #pragma unroll
-      for (int l = 0; l < 4; l++) acc[l] += (uint64_t)v[j] * cf[j][l];
-      if (j % 3 == 2) {
+        for (int j = 0; j < 16; j++) {
+          const uint4 cf = cs[(jb + j) * 64 + lane];
+          uint64_t* a = acc[j & 1];
+          a[0] += (uint64_t)v[j] * cf.x;
+          a[1] += (uint64_t)v[j] * cf.y;
+          a[2] += (uint64_t)v[j] * cf.z;
+          a[3] += (uint64_t)v[j] * cf.w;
+          if ((j >> 1) == 2 || (j >> 1) == 5 || (j >> 1) == 7) {  // at most three products per set between folds
 #pragma unroll
-        for (int l = 0; l < 4; l++) acc[l] = fold(acc[l]);
+            for (int l = 0; l < 4; l++) a[l] = fold(a[l]);
+          }
+        }
       }
     }
+    uint64_t accs[4];
+#pragma unroll
+    for (int l = 0; l < 4; l++) accs[l] = fold(acc[0][l]) + fold(acc[1][l]);
     // Across the wave: each step adds two values below 2^61 and folds.
 #pragma unroll
     for (int l = 0; l < 4; l++) {
-      uint64_t x = fold(acc[l]);
+      uint64_t x = fold(accs[l]);
 #pragma unroll
       for (int off = 32; off >= 1; off >>= 1) x = fold(x + __shfl_xor(x, off));
-      if (lane == 0) red[wave][l] = x;
+      accs[l] = x;
     }
-    __syncthreads();
-    if (tid < 4) {
-      uint64_t x = 0;
-#pragma unroll
-      for (int w = 0; w < RLC_NT / 64; w++) x = fold(x + red[w][tid]);
-      partial[((size_t)(r0 + i) * nblk + blk) * 4 + tid] = mod_p(x);
+    if (lane == 0) {
+      uint4 o;
+      o.x = mod_p(accs[0]);
+      o.y = mod_p(accs[1]);
+      o.z = mod_p(accs[2]);
+      o.w = mod_p(accs[3]);
+      reinterpret_cast<uint4*>(partial)[(size_t)(r0 + i) * nblk + blk] = o;
     }
-    __syncthreads();
   }
 }
 
@@ -157,7 +169,7 @@ extern "C" int zoda_hip_rlc_rows(const uint32_t* d_square, uint32_t rows, uint32
   uint32_t* partial = nullptr;
   ZODA_HIP(hipMallocAsync((void**)&partial, (size_t)count * nblk * 4 * sizeof(uint32_t), s));
   hipLaunchKernelGGL(rlc_rows_kernel, dim3(nblk, nrb), dim3(RLC_NT), 0, s, d_square, rows, cols, d_row_index,
-                     count, d_coeffs, partial);
+                     count, reinterpret_cast<const uint4*>(d_coeffs), partial);
   int rc = 0;
   if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "RLC kernel launch failed");
   if (!rc) {
