@@ -211,6 +211,14 @@ int zoda_hip_rlc_rows(const uint32_t* d_square, uint32_t rows, uint32_t cols, co
  * to hex_out (65 bytes).  Runs on the host (one sequential hash chain). */
 int zoda_data_root(const uint32_t* values, size_t count, char* hex_out);
 
+/* Generic-modulus transforms: ntt / intt of src/ntt/cpu.rs:1-32,45-60 over
+ * any modulus in [2, 2^31) with the caller's n roots (roots[i] = w^i), with
+ * the reference's exact arithmetic (including its n^(m-2) inverse of n).
+ * Host memory, `polys` polynomials `stride` u32 apart, the first n (a power
+ * of two <= 8192) of each transformed in place; inverse != 0 for intt. */
+int zoda_hip_ntt_mod(uint32_t* values, const uint32_t* roots, uint32_t n, uint32_t polys, uint32_t stride,
+                     uint32_t modulus, int inverse);
+
 /* ---- Erasure decoding ----
  *
  * The data square back from any k of the 2k rows of its extended square

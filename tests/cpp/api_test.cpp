@@ -128,12 +128,38 @@ static void test_encode_validated(size_t side) {
   std::printf("ok test_encode_validated side=%zu\n", side);
 }
 
+// The dense data square against the reference's cell-list semantics: filled
+// column by column (as the encoder fills it), in any order of growth, its rows,
+// columns and root equal those of the reference square; the root is printed so
+// the Python side can check it against hashlib.
+static void test_data_square() {
+  const size_t cols = 7, rows = 5;
+  zoda::DataSquare sq;
+  for (size_t c = 0; c < cols; c++)
+    for (size_t r = 0; r < rows; r++) sq.set_cell(c, r, (uint32_t)((r * cols + c) * 2654435761u % zoda::BABYBEAR_PRIME));
+  EXPECT(sq.columns() == cols && sq.rows() == rows, "shape");
+  for (size_t r = 0; r < rows; r++) {
+    std::vector<uint32_t> row = sq.get_row(r);
+    for (size_t c = 0; c < cols; c++) EXPECT(row[c] == sq.get_cell(c, r), "row %zu col %zu", r, c);
+  }
+  std::vector<uint32_t> col3 = sq.get_column(3);
+  for (size_t r = 0; r < rows; r++) EXPECT(col3[r] == (uint32_t)((r * cols + 3) * 2654435761u % zoda::BABYBEAR_PRIME), "col");
+  // built in another order (rows first, growing both ways): same cells, same root
+  zoda::DataSquare sq2(1, 1);
+  for (size_t r = rows; r-- > 0;)
+    for (size_t c = cols; c-- > 0;) sq2.set_cell(c, r, sq.get_cell(c, r));
+  EXPECT(sq2.hash_root() == sq.hash_root(), "root depends on the build order");
+  std::printf("data_square_root %s\n", sq.hash_root().c_str());
+  std::printf("ok test_data_square\n");
+}
+
 int main() {
   EXPECT(zoda::gpu_available(), "no HIP device");
   test_ntt_intt_vs_cpu();
   test_roundtrip();
   test_many_columns_and_strided();
   test_bad_arguments();
+  test_data_square();
   for (size_t side : {4, 64, 256, 1024}) test_encode_validated(side);
   std::printf("all ok\n");
   return 0;

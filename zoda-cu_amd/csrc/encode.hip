@@ -62,7 +62,7 @@ __global__ __launch_bounds__(PL::NT) void encode_pipelined(const uint32_t* __res
   extern __shared__ int32_t lds[];
   encode_load_tables<PL>((int)threadIdx.x, lds, T);
   const uint32_t* sb = reinterpret_cast<const uint32_t*>(lds + PL::TILE_WORDS);
-  const TwTab inv1{T.inv1, sb + T.o_inv1};
+  const TwTab inv1{T.inv1, PL::TWL ? sb + T.o_inv1 : nullptr};
   const uint32_t k = (uint32_t)PL::S;
   uint32_t raw[PP::G][PP::RR];
   uint32_t l = blockIdx.x;

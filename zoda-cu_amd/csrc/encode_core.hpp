@@ -352,7 +352,7 @@ ZHD void encode_step(int tid, uint32_t tile, const uint32_t* __restrict__ in,
 }
 
 // Register pipeline of the first pass for whole tiles in the row layout
-// (plans with D >= 2 and LDS twiddles).  load() brings a tile's first-pass
+// (plans with D >= 2).  load() brings a tile's first-pass
 // values into registers -- issued while the previous tile is still being
 // transformed, so the loads are in flight behind that work -- first()
 // transforms them into the LDS tile, and store_even() writes them to the even
@@ -362,7 +362,7 @@ template <class PL>
 struct Pipe {
   static constexpr int R = PL::R1, RR = 1 << R, LO = PL::LO1;
   static constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
-  static_assert(PL::D >= 2 && PL::TWL && (PL::W << (PL::L - R)) % PL::NT == 0, "pipeline shape");
+  static_assert(PL::D >= 2 && (PL::W << (PL::L - R)) % PL::NT == 0, "pipeline shape");
 
   static ZHD void load(int tid, uint32_t tile, const uint32_t* __restrict__ in, uint32_t cols,
                        uint32_t (&raw)[G][RR]) {
@@ -403,11 +403,11 @@ struct Pipe {
       const uint32_t base = u << R;  // u < 2^LO for the top digit
       if constexpr (PL::D == 2) {
 #pragma unroll
-        for (int k = 0; k < RR; k++) x[k] = mulmod(x[k], tw_get<true>(inv1, base + k));
+        for (int k = 0; k < RR; k++) x[k] = mulmod(x[k], tw_get<PL::TWL>(inv1, base + k));
       } else {
         x[0] = reduce(x[0]);
 #pragma unroll
-        for (int k = 1; k < RR; k++) x[k] = mulmod(x[k], tw_get<true>(inv1, base + k));
+        for (int k = 1; k < RR; k++) x[k] = mulmod(x[k], tw_get<PL::TWL>(inv1, base + k));
       }
 #pragma unroll
       for (int m = 0; m < RR; m++) lds[row_of<LO, R>(u, m) * PL::W + c] = (int32_t)x[m];

@@ -15,6 +15,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
+#include <algorithm>
 #include <string>
 #include <utility>
 #include <vector>
@@ -142,6 +143,59 @@ inline void encode_gpu(const EncodingConfig& cfg, const uint32_t* input, PinnedS
                              ZODA_LAYOUT_COL),
         "zoda_hip_encode_host");
 }
+
+// Dense replacement of BabyBearDataSquare (src/zoda/babybear.rs:49-112):
+// the same accessors over one column-major array instead of a list of cells
+// searched linearly (which makes building an s x s square O(s^4)).  Cells
+// never set read as 0; the reference's square has them missing, which only
+// differs for squares that are not filled.  hash_root() hashes the cells
+// column by column -- the reference hashes in insertion order, and the
+// encoder fills its square column by column (babybear.rs:156-160), so the
+// roots agree for it.
+class DataSquare {
+ public:
+  DataSquare() = default;
+  DataSquare(size_t columns, size_t rows) : columns_(columns), rows_(rows), v_(columns * rows, 0u) {}
+
+  void set_cell(size_t column, size_t row, uint32_t value) {
+    if (column >= columns_ || row >= rows_) grow(std::max(columns_, column + 1), std::max(rows_, row + 1));
+    v_[column * rows_ + row] = value;
+  }
+  uint32_t get_cell(size_t column, size_t row) const {
+    return column < columns_ && row < rows_ ? v_[column * rows_ + row] : 0u;
+  }
+  std::vector<uint32_t> get_row(size_t row) const {
+    std::vector<uint32_t> out(columns_);
+    for (size_t c = 0; c < columns_; c++) out[c] = get_cell(c, row);
+    return out;
+  }
+  std::vector<uint32_t> get_column(size_t column) const {
+    std::vector<uint32_t> out(rows_);
+    for (size_t r = 0; r < rows_; r++) out[r] = get_cell(column, r);
+    return out;
+  }
+  // SHA-256 of the cells' little-endian bytes, column by column, as hex.
+  std::string hash_root() const {
+    char hex[65];
+    check(zoda_data_root(v_.data(), v_.size(), hex), "zoda_data_root");
+    return std::string(hex);
+  }
+  size_t columns() const { return columns_; }
+  size_t rows() const { return rows_; }
+  const uint32_t* data() const { return v_.data(); }
+
+ private:
+  void grow(size_t columns, size_t rows) {
+    std::vector<uint32_t> v(columns * rows, 0u);
+    for (size_t c = 0; c < columns_; c++)
+      for (size_t r = 0; r < rows_; r++) v[c * rows + r] = v_[c * rows_ + r];
+    v_.swap(v);
+    columns_ = columns;
+    rows_ = rows;
+  }
+  size_t columns_ = 0, rows_ = 0;
+  std::vector<uint32_t> v_;
+};
 
 // Device-resident encode (asynchronous on `stream`, a hipStream_t or nullptr).
 inline void encode_device(const uint32_t* d_in, uint32_t k, uint32_t cols, uint32_t n, uint32_t* d_out,

@@ -74,6 +74,8 @@ def _load():
         "zoda_hip_rlc_rows": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp,
                                              _vp]),
         "zoda_data_root": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_char_p]),
+        "zoda_hip_ntt_mod": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_uint32, ctypes.c_int]),
         "zoda_hip_decode_device": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp,
                                                   _vp]),
         "zoda_hip_batch_plan_create": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(_vp)]),
@@ -488,3 +490,17 @@ def decode_tensor(encoded, known_rows, out=None):
                                       torch.cuda.current_stream().cuda_stream),
            "zoda_hip_decode_device")
     return out
+
+
+def ntt_mod(values, roots, modulus: int, n: int | None = None, stride: int | None = None, inverse: bool = False):
+    """ntt / intt of src/ntt/cpu.rs over any modulus < 2^31 with the caller's
+    roots, on the GPU; transforms the first n of every `stride` values of a
+    host uint32 array in place and returns it."""
+    v = np.ascontiguousarray(values, dtype=np.uint32)
+    r = np.ascontiguousarray(roots, dtype=np.uint32)
+    n = len(r) if n is None else n
+    stride = n if stride is None else stride
+    assert len(r) == n and v.size % stride == 0
+    _check(lib.zoda_hip_ntt_mod(v.ctypes.data, r.ctypes.data, n, v.size // stride, stride, modulus,
+                                1 if inverse else 0), "zoda_hip_ntt_mod")
+    return v
