@@ -2,7 +2,8 @@
 // side s = 2^L.  Tiles are LDS-resident as int32 (W * s * 4 bytes <= 128 KiB);
 // sides up to 32 keep a whole column in one thread's registers (no LDS).  The
 // flags keep the twiddle tables in LDS as u32 next to the tile, and run whole
-// tiles in a persistent kernel that prefetches each next tile into LDS.
+// tiles in a persistent kernel whose first pass reads registers loaded while
+// the previous tile was transformed.
 #pragma once
 
 #include "encode_core.hpp"
