@@ -275,7 +275,7 @@ def rlc_leg(zoda_hip, torch, t_out, reps: int = 10):
 def decode_leg(zoda_hip, torch, t_in, t_out, seed: int, reps: int = 3):
     """Erasure decode of the square from a random half of its 2s rows (the
     data back, bit-exact), wall clock around the synchronous call (it includes
-    the O(s^2) host set-up of the erasure pattern), median of `reps`."""
+    the set-up of the erasure pattern), median of `reps`."""
     import numpy as np
     side, cols = t_in.shape
     known = np.random.default_rng(seed).choice(2 * side, size=side, replace=False)

@@ -29,7 +29,7 @@ def patterns(k, rng):
     yield "random", rng.choice(n, size=k, replace=False)
 
 
-@pytest.mark.parametrize("k,cols", [(1, 5), (2, 33), (64, 1000), (256, 4097), (1024, 300), (4096, 64)])
+@pytest.mark.parametrize("k,cols", [(1, 5), (2, 33), (64, 1000), (256, 4097), (1024, 300), (4096, 64), (16384, 3)])
 def test_decode_from_any_k_rows(zh, torch, k, cols):
     rng = np.random.default_rng(k * 7 + cols)
     t_in = torch.empty((k, cols), dtype=torch.int32, device="cuda")
@@ -71,3 +71,5 @@ def test_decode_argument_errors(zh, torch):
         zh.decode_tensor(enc, np.array([0, 1, 2, 3, 4, 5, 6, 16]))   # outside the 16 rows
     with pytest.raises(zh.ZodaError):
         zh.decode_tensor(torch.zeros((12, 8), dtype=torch.int32, device="cuda"), np.arange(6))  # k = 6
+    with pytest.raises(zh.ZodaError):
+        zh.decode_tensor(torch.zeros((65536, 1), dtype=torch.int32, device="cuda"), np.arange(32768))  # k > 16384

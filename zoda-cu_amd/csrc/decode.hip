@@ -4,21 +4,27 @@
 // roots of unity w^r of a polynomial f of degree < k, with the data at the
 // even rows (src/zoda/babybear.rs:195-211).
 //
-// Per column, with K the k known rows and E the others (k of them):
+// Per column, with K the k known rows and E the others (k of them), n = 2k:
 //   Z(x) = prod_{e in E} (x - w^e)                  vanishes on the erasures
-//   y_r  = out[r] * Z(w^r) on K, 0 on E              = g(w^r), g = f * Z, deg g < 2k
-//   G    = INTT_2k(y)                                coefficients of g
-//   g(h w^r) = NTT_2k(G_i h^i)                       on a coset where Z != 0 (h = 31)
+//   y_r  = out[r] * Z(w^r) on K, 0 on E              = g(w^r), g = f * Z, deg g < n
+//   G    = INTT_n(y)                                 coefficients of g
+//   g(h w^r) = NTT_n(G_i h^i)                        on a coset where Z != 0 (h = 31)
 //   f(h w^r) = g(h w^r) / Z(h w^r)
-//   f_i  = INTT_2k(f(h w^r))_i * h^-i, i < k         (the upper k are zero)
+//   f_i  = INTT_n(f(h w^r))_i * h^-i, i < k          (the upper k are zero)
 //   data[j] = f(w^(2j)) = NTT_k(f)_j
-// Z on K and 1/Z on the coset depend only on which rows are known: they are
-// computed once per call on the host (O(k^2)).  On the device the square goes
-// column-major into a scratch buffer (tiled transpose, scaled on the way),
-// through the four batched transforms of ntt_poly.hip with pointwise scalings
-// between them, and back row-major.
+//
+// Device work, all in canonical u32 with Montgomery multiplications:
+//   dec_prep    Z(w^r) on K and 1/Z(h w^r) for every r (n threads, k factors
+//               each): the only O(k^2) part, once per call.
+//   dec_gather  row-major square -> column-major scratch, times Z(w^r)
+//               (tiled transpose; erased rows are never read).
+//   dec_fused   one work-group per column, in LDS: INTT_n (decimation in time,
+//               bit-reversed in, natural out), * h^i/n, NTT_n (decimation in
+//               frequency, natural in, bit-reversed out), * 1/Z(h w^r) in the
+//               same bit-reversed order, INTT_n, * h^-i/n, NTT_k -- no
+//               permutation passes, one read and one write of the scratch.
+//   dec_scatter column-major -> row-major data (tiled transpose).
 #include <algorithm>
-#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -28,28 +34,53 @@ namespace zoda {
 namespace {
 
 constexpr uint32_t DEC_COSET = 31;  // a generator: h w^r is never a 2k-th root of unity
-constexpr int TT = 32;               // transpose tile
+constexpr uint32_t DEC_MAX_K = 16384;  // n = 2k values of one column in LDS: <= 128 KiB
+constexpr int TT = 32;              // transpose tile
+constexpr int DEC_NT = 256;
 
-uint32_t mulp(uint32_t a, uint32_t b) { return (uint32_t)((uint64_t)a * b % P); }
+constexpr uint32_t MONT_ONE = (uint32_t)((1ull << 32) % P);  // 1 in Montgomery form
 
-// Multiplier w with its Shoup companion floor(w * 2^32 / p).
-struct Shoup {
-  uint32_t w, wp;
-};
-
-Shoup shoup(uint32_t w) { return Shoup{w, (uint32_t)(((uint64_t)w << 32) / P)}; }
-
-// a * s.w mod p, canonical, for any 32-bit a (Shoup's method: the quotient
-// estimate mulhi(a, wp) is exact or one short).
-__device__ __forceinline__ uint32_t mul_shoup(uint32_t a, Shoup s) {
-  const uint32_t q = __umulhi(a, s.wp);
-  const uint32_t r = a * s.w - q * P;  // in [0, 2p), mod 2^32
-  return r >= P ? r - P : r;
+__device__ __forceinline__ uint32_t brev_n(uint32_t i, uint32_t log_n) {
+  return log_n ? (__brev(i) >> (32 - log_n)) : 0u;
 }
 
-// T[c * n + r] = enc[r * cols + c] * z[r] for r < n (a 32 x 32 tile per block).
+// Montgomery form a^e.
+__device__ uint32_t mont_pow(uint32_t a, uint32_t e) {
+  uint32_t r = MONT_ONE;
+  while (e) {
+    if (e & 1) r = mont_mul(r, a);
+    a = mont_mul(a, a);
+    e >>= 1;
+  }
+  return r;
+}
+
+// x_m[r] = w^r in Montgomery form; ep_m: the erased points, Montgomery form.
+// zk_m[r] = Z(w^r) (0 on erased rows); zinv_br_m[i] = 1 / Z(h w^(brev(i))).
+__global__ void dec_prep(const uint32_t* __restrict__ x_m, const uint8_t* __restrict__ known,
+                         const uint32_t* __restrict__ ep_m, uint32_t ne, uint32_t log_n, uint32_t h_m,
+                         uint32_t* __restrict__ zk_m, uint32_t* __restrict__ zinv_br_m) {
+  const uint32_t n = 1u << log_n;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * n) return;
+  const bool coset = t >= n;
+  const uint32_t r = coset ? brev_n(t - n, log_n) : t;
+  if (!coset && !known[r]) {
+    zk_m[r] = 0;
+    return;
+  }
+  const uint32_t x = coset ? mont_mul(h_m, x_m[r]) : x_m[r];
+  uint32_t acc = MONT_ONE;
+  for (uint32_t j = 0; j < ne; j++) acc = mont_mul(acc, sub_mod(x, ep_m[j]));
+  if (coset)
+    zinv_br_m[t - n] = mont_pow(acc, P - 2);
+  else
+    zk_m[r] = acc;
+}
+
+// T[c * n + r] = enc[r * cols + c] * Z(w^r) (a 32 x 32 tile per block).
 __global__ void dec_gather(const uint32_t* __restrict__ enc, uint32_t cols, uint32_t n,
-                           const Shoup* __restrict__ z, uint32_t* __restrict__ T) {
+                           const uint32_t* __restrict__ zk_m, uint32_t* __restrict__ T) {
   __shared__ uint32_t tile[TT][TT + 1];
   const uint32_t c0 = blockIdx.x * TT, r0 = blockIdx.y * TT;
   const uint32_t tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
@@ -57,8 +88,8 @@ __global__ void dec_gather(const uint32_t* __restrict__ enc, uint32_t cols, uint
     const uint32_t r = r0 + i, c = c0 + tx;
     uint32_t v = 0;
     if (r < n && c < cols) {
-      const Shoup s = z[r];
-      v = s.w ? mul_shoup(enc[(size_t)r * cols + c], s) : 0u;  // erased rows are not read
+      const uint32_t z = zk_m[r];
+      v = z ? mont_mul(enc[(size_t)r * cols + c], z) : 0u;  // erased rows are not read
     }
     tile[i][tx] = v;
   }
@@ -69,15 +100,64 @@ __global__ void dec_gather(const uint32_t* __restrict__ enc, uint32_t cols, uint
   }
 }
 
-// T[c * stride + i] *= w[i] for i < len.
-__global__ void dec_scale(uint32_t* __restrict__ T, uint32_t stride, uint32_t len, uint32_t polys,
-                          const Shoup* __restrict__ w) {
-  const size_t total = (size_t)polys * len;
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t c = (uint32_t)(t / len), i = (uint32_t)(t % len);
-    uint32_t* p = T + (size_t)c * stride + i;
-    *p = mul_shoup(*p, w[i]);
+// Radix-2 decimation in time over v[0, 1 << log_n): bit-reversed in, natural
+// out; tw[h + j] = (root of order 2h)^j in Montgomery form.
+__device__ __forceinline__ void dit(uint32_t* v, uint32_t log_n, const uint32_t* __restrict__ tw) {
+  const uint32_t half_n = (1u << log_n) >> 1;
+  for (uint32_t lh = 0; lh < log_n; lh++) {
+    const uint32_t h = 1u << lh;
+    for (uint32_t b = threadIdx.x; b < half_n; b += DEC_NT) {
+      const uint32_t j = b & (h - 1), base = ((b >> lh) << (lh + 1)) + j;
+      const uint32_t u = v[base], w = mont_mul(v[base + h], tw[h + j]);
+      v[base] = add_mod(u, w);
+      v[base + h] = sub_mod(u, w);
+    }
+    __syncthreads();
   }
+}
+
+// Radix-2 decimation in frequency: natural in, bit-reversed out.
+__device__ __forceinline__ void dif(uint32_t* v, uint32_t log_n, const uint32_t* __restrict__ tw) {
+  const uint32_t half_n = (1u << log_n) >> 1;
+  for (uint32_t lh = log_n; lh-- > 0;) {
+    const uint32_t h = 1u << lh;
+    for (uint32_t b = threadIdx.x; b < half_n; b += DEC_NT) {
+      const uint32_t j = b & (h - 1), base = ((b >> lh) << (lh + 1)) + j;
+      const uint32_t u = v[base], w = v[base + h];
+      v[base] = add_mod(u, w);
+      v[base + h] = mont_mul(sub_mod(u, w), tw[h + j]);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ void scale(uint32_t* v, uint32_t len, const uint32_t* __restrict__ m) {
+  for (uint32_t i = threadIdx.x; i < len; i += DEC_NT) v[i] = mont_mul(v[i], m[i]);
+  __syncthreads();
+}
+
+// One work-group per column of T (n values); leaves the k data values of the
+// column, in order, at T[c * n + j].
+__global__ __launch_bounds__(DEC_NT) void dec_fused(uint32_t* __restrict__ T, uint32_t log_n,
+                                                    const uint32_t* __restrict__ fwd_n,
+                                                    const uint32_t* __restrict__ inv_n,
+                                                    const uint32_t* __restrict__ fwd_k,
+                                                    const uint32_t* __restrict__ hpow_m,
+                                                    const uint32_t* __restrict__ zinv_br_m,
+                                                    const uint32_t* __restrict__ hinv_m) {
+  extern __shared__ uint32_t v[];
+  const uint32_t n = 1u << log_n, k = n >> 1;
+  uint32_t* col = T + (size_t)blockIdx.x * n;
+  for (uint32_t i = threadIdx.x; i < n; i += DEC_NT) v[brev_n(i, log_n)] = col[i];
+  __syncthreads();
+  dit(v, log_n, inv_n);            // n * G, natural order
+  scale(v, n, hpow_m);             // * h^i / n
+  dif(v, log_n, fwd_n);            // g(h w^r) at position brev(r)
+  scale(v, n, zinv_br_m);          // f(h w^r), same positions
+  dit(v, log_n, inv_n);            // n * f_i h^i, natural order
+  scale(v, k, hinv_m);             // f_i, i < k
+  dif(v, log_n - 1, fwd_k);        // f(w_k^j) at position brev_k(j)
+  for (uint32_t j = threadIdx.x; j < k; j += DEC_NT) col[j] = v[brev_n(j, log_n - 1)];
 }
 
 // data[j * cols + c] = T[c * stride + j] for j < k.
@@ -97,60 +177,6 @@ __global__ void dec_scatter(const uint32_t* __restrict__ T, uint32_t stride, uin
   }
 }
 
-// The per-row vectors of one decode (host, O(n k), split over threads).
-struct DecodeVectors {
-  std::vector<Shoup> zk;    // Z(w^r) on known rows, 0 on erased ones
-  std::vector<Shoup> zinv;  // 1 / Z(h w^r)
-  std::vector<Shoup> hpow;  // h^i, i < n
-  std::vector<Shoup> hinv;  // h^-i, i < k
-};
-
-void decode_vectors(uint32_t k, const std::vector<uint8_t>& known, DecodeVectors* v) {
-  const uint32_t n = 2 * k;
-  const uint32_t w = root_of_unity(log2_ceil(n));
-  std::vector<uint32_t> x(n);
-  x[0] = 1;
-  for (uint32_t r = 1; r < n; r++) x[r] = mulp(x[r - 1], w);
-  std::vector<uint32_t> erased;
-  for (uint32_t r = 0; r < n; r++)
-    if (!known[r]) erased.push_back(x[r]);
-  std::vector<uint32_t> zk(n), zc(n);
-  const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  std::vector<std::thread> th;
-  for (unsigned t = 0; t < nth; t++)
-    th.emplace_back([&, t] {
-      for (uint32_t r = t; r < n; r += nth) {
-        const uint32_t hx = mulp(DEC_COSET, x[r]);
-        uint32_t a = 1, b = 1;
-        for (uint32_t e : erased) {
-          if (known[r]) a = mulp(a, x[r] + P - e);
-          b = mulp(b, hx + P - e);
-        }
-        zk[r] = known[r] ? a : 0;
-        zc[r] = b;
-      }
-    });
-  for (auto& t : th) t.join();
-  v->zk.resize(n);
-  v->zinv.resize(n);
-  v->hpow.resize(n);
-  v->hinv.resize(k);
-  const uint32_t hi = inv_mod_host(DEC_COSET);
-  uint32_t hp = 1, hq = 1;
-  for (uint32_t r = 0; r < n; r++) {
-    v->zk[r] = shoup(zk[r]);
-    v->zinv[r] = shoup(inv_mod_host(zc[r]));
-    v->hpow[r] = shoup(hp);
-    hp = mulp(hp, DEC_COSET);
-    if (r < k) {
-      v->hinv[r] = shoup(hq);
-      hq = mulp(hq, hi);
-    }
-  }
-}
-
-unsigned dec_grid(size_t n) { return (unsigned)std::min<size_t>((n + 255) / 256, 1u << 16); }
-
 }  // namespace
 
 }  // namespace zoda
@@ -160,9 +186,9 @@ using namespace zoda;
 extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_t cols, const uint32_t* known_rows,
                                       uint32_t nknown, uint32_t* d_data, void* stream) {
   clear_error();
-  if (!d_enc || !d_data || !known_rows || cols == 0 || k == 0 || !is_pow2(k) || k > (1u << 15))
-    return fail(ZODA_EINVAL, "bad arguments (k must be a power of two <= 2^15)");
-  const uint32_t n = 2 * k;
+  if (!d_enc || !d_data || !known_rows || cols == 0 || !is_pow2(k) || k > DEC_MAX_K)
+    return fail(ZODA_EINVAL, "bad arguments (k must be a power of two <= %u)", DEC_MAX_K);
+  const uint32_t n = 2 * k, log_n = log2_ceil(n);
   std::vector<uint8_t> known(n, 0);
   uint32_t have = 0;
   for (uint32_t i = 0; i < nknown && have < k; i++) {
@@ -176,63 +202,73 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
   if (have < k) return fail(ZODA_EINVAL, "%u distinct known rows, %u needed", have, k);
   DeviceState* st = nullptr;
   ZODA_TRY(current_state(&st));
-  DecodeVectors v;
-  decode_vectors(k, known, &v);
   MontTables tn, tk;
-  ZODA_TRY(get_mont_tables(st, log2_ceil(n), &tn));
-  if (k > 1) ZODA_TRY(get_mont_tables(st, log2_ceil(k), &tk));
+  ZODA_TRY(get_mont_tables(st, log_n, &tn));
+  ZODA_TRY(get_mont_tables(st, log_n - 1, &tk));
 
+  // O(n) host vectors, Montgomery form: the points w^r, the erased points,
+  // h^i / n and h^-i / n.
+  const uint32_t w = root_of_unity(log_n);
+  const uint32_t n_inv = inv_mod_host(n), h_inv = inv_mod_host(DEC_COSET);
+  std::vector<uint32_t> x_m(n), ep_m, hpow_m(n), hinv_m(k);
+  uint32_t x = 1, hp = n_inv, hq = n_inv;
+  for (uint32_t r = 0; r < n; r++) {
+    x_m[r] = to_mont_host(x);
+    if (!known[r]) ep_m.push_back(x_m[r]);
+    hpow_m[r] = to_mont_host(hp);
+    if (r < k) hinv_m[r] = to_mont_host(hq);
+    x = mul_mod_host(x, w);
+    hp = mul_mod_host(hp, DEC_COSET);
+    hq = mul_mod_host(hq, h_inv);
+  }
+  const uint32_t ne = (uint32_t)ep_m.size();  // = k
+
+  // One device block: x_m | hpow_m | hinv_m | ep_m | zk_m | zinv_br_m | known.
   hipStream_t s = (hipStream_t)stream;
-  Shoup* d_vec = nullptr;
+  const size_t up_words = (size_t)n + n + k + ne;
+  const size_t words = up_words + n + n;
+  uint32_t* d_blk = nullptr;
   uint32_t* T = nullptr;
-  const size_t nvec = 3 * (size_t)n + k;
-  ZODA_HIP(hipMalloc((void**)&d_vec, nvec * sizeof(Shoup)));
-  std::vector<Shoup> all;
-  all.reserve(nvec);
-  all.insert(all.end(), v.zk.begin(), v.zk.end());
-  all.insert(all.end(), v.hpow.begin(), v.hpow.end());
-  all.insert(all.end(), v.zinv.begin(), v.zinv.end());
-  all.insert(all.end(), v.hinv.begin(), v.hinv.end());
+  ZODA_HIP(hipMalloc((void**)&d_blk, words * 4 + n));
+  uint32_t* d_x = d_blk;
+  uint32_t* d_hpow = d_x + n;
+  uint32_t* d_hinv = d_hpow + n;
+  uint32_t* d_ep = d_hinv + k;
+  uint32_t* d_zk = d_ep + ne;
+  uint32_t* d_zinv = d_zk + n;
+  uint8_t* d_known = reinterpret_cast<uint8_t*>(d_zinv + n);
+  std::vector<uint32_t> host;
+  host.reserve(up_words);
+  host.insert(host.end(), x_m.begin(), x_m.end());
+  host.insert(host.end(), hpow_m.begin(), hpow_m.end());
+  host.insert(host.end(), hinv_m.begin(), hinv_m.end());
+  host.insert(host.end(), ep_m.begin(), ep_m.end());
   int rc = 0;
-  hipError_t e = hipMemcpy(d_vec, all.data(), nvec * sizeof(Shoup), hipMemcpyHostToDevice);
-  if (e != hipSuccess) rc = fail((int)e, "decode vectors upload failed: %s", hipGetErrorString(e));
-  if (!rc) {
-    e = hipMallocAsync((void**)&T, (size_t)cols * n * sizeof(uint32_t), s);
-    if (e != hipSuccess) rc = fail((int)e, "decode scratch allocation failed: %s", hipGetErrorString(e));
+  if (hipMemcpy(d_blk, host.data(), up_words * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_known, known.data(), n, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(ZODA_ESTATE, "decode vectors upload failed");
+  if (!rc && hipMallocAsync((void**)&T, (size_t)cols * n * sizeof(uint32_t), s) != hipSuccess) {
+    T = nullptr;
+    rc = fail(ZODA_ENOMEM, "decode scratch allocation failed");
   }
-  const Shoup* zk = d_vec;
-  const Shoup* hpow = d_vec + n;
-  const Shoup* zinv = d_vec + 2 * n;
-  const Shoup* hinv = d_vec + 3 * n;
+  const size_t lds = (size_t)n * sizeof(uint32_t);
+  if (!rc && lds > 65536 &&
+      hipFuncSetAttribute((const void*)dec_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+    rc = fail(ZODA_ESTATE, "raising the decoder's LDS limit failed");
   if (!rc) {
+    hipLaunchKernelGGL(dec_prep, dim3((2 * n + 255) / 256), dim3(256), 0, s, d_x, d_known, d_ep, ne, log_n,
+                       to_mont_host(DEC_COSET), d_zk, d_zinv);
     hipLaunchKernelGGL(dec_gather, dim3((cols + TT - 1) / TT, (n + TT - 1) / TT), dim3(TT, 8), 0, s, d_enc, cols,
-                       n, zk, T);
-    if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "decode gather launch failed");
-  }
-  if (!rc) rc = launch_ntt_poly(T, log2_ceil(n), cols, n, 1, tn, s);
-  if (!rc) {
-    hipLaunchKernelGGL(dec_scale, dim3(dec_grid((size_t)cols * n)), dim3(256), 0, s, T, n, n, cols, hpow);
-    if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "decode scale launch failed");
-  }
-  if (!rc) rc = launch_ntt_poly(T, log2_ceil(n), cols, n, 0, tn, s);
-  if (!rc) {
-    hipLaunchKernelGGL(dec_scale, dim3(dec_grid((size_t)cols * n)), dim3(256), 0, s, T, n, n, cols, zinv);
-    if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "decode scale launch failed");
-  }
-  if (!rc) rc = launch_ntt_poly(T, log2_ceil(n), cols, n, 1, tn, s);
-  if (!rc) {
-    hipLaunchKernelGGL(dec_scale, dim3(dec_grid((size_t)cols * k)), dim3(256), 0, s, T, n, k, cols, hinv);
-    if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "decode scale launch failed");
-  }
-  if (!rc && k > 1) rc = launch_ntt_poly(T, log2_ceil(k), cols, n, 0, tk, s);
-  if (!rc) {
+                       n, d_zk, T);
+    hipLaunchKernelGGL(dec_fused, dim3(cols), dim3(DEC_NT), lds, s, T, log_n, tn.fwd, tn.inv,
+                       tk.fwd, d_hpow, d_zinv, d_hinv);
     hipLaunchKernelGGL(dec_scatter, dim3((k + TT - 1) / TT, (cols + TT - 1) / TT), dim3(TT, 8), 0, s, T, n, k, cols,
                        d_data);
-    if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "decode scatter launch failed");
+    if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "decode launch failed");
   }
   if (T && hipFreeAsync(T, s) != hipSuccess && !rc) rc = fail(ZODA_ESTATE, "hipFreeAsync failed");
-  // d_vec is read by the kernels just queued: free it once they are done.
+  // d_blk is read by the kernels just queued: free it once they are done.
   if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(ZODA_ESTATE, "decode failed");
-  (void)hipFree(d_vec);
+  (void)hipFree(d_blk);
   return rc;
 }
