@@ -54,10 +54,24 @@ static void run_late_steps(uint32_t tile, const uint32_t* in, uint32_t* out, uin
   if constexpr (STEP + 1 < PL::STEPS) run_late_steps<PL, STEP + 1>(tile, in, out, cols, lds, T);
 }
 
-// The pipelined persistent kernel's order of work on one work-group: the
-// first pass from the registers loaded ahead, the second pass with the
-// even-row stores of those registers, the loads of the next tile into the
-// same registers, the remaining passes.
+// The pipelined persistent kernel's order of work on one work-group
+// (encode_pipelined): the first pass from the registers loaded ahead with the
+// even-row stores of those registers, the middle passes with the loads of the
+// next tile into the same registers in parts, the last pass.
+template <class PL, int STEP>
+static void run_step_loading(uint32_t tile, const uint32_t* in, uint32_t* out, uint32_t cols, int32_t* lds,
+                             const EncodeTabs& T, bool more, uint32_t next, int part0, int np,
+                             std::vector<uint32_t>* raw, int words) {
+  using PP = Pipe<PL>;
+  for (int tid = 0; tid < PL::NT; tid++)
+    encode_step<PL, STEP, true>(tid, tile, in, out, (uint32_t)PL::S, cols, 0, lds, T, [&](int gi) {
+      if (more)
+        PP::load_part(tid, next, in, cols,
+                      *reinterpret_cast<uint32_t(*)[PP::G][PP::RR]>(raw->data() + (size_t)tid * words), part0 + gi,
+                      np);
+    });
+}
+
 template <class PL>
 static int sim_pipelined(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out) {
   using PP = Pipe<PL>;
@@ -71,19 +85,25 @@ static int sim_pipelined(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t
   for (int tid = 0; tid < PL::NT; tid++) encode_load_tables<PL>(tid, lds.data(), T);
   const uint32_t* sb = reinterpret_cast<const uint32_t*>(lds.data() + PL::TILE_WORDS);
   const TwTab inv1{T.inv1, sb + T.o_inv1};
-  struct Regs { uint32_t v[PP::G][PP::RR]; };
-  std::vector<Regs> raw(PL::NT);
+  constexpr int WORDS = PP::G * PP::RR;
+  using Raw = uint32_t[PP::G][PP::RR];
+  std::vector<uint32_t> regs((size_t)PL::NT * WORDS);
+  auto raw = [&](int tid) -> Raw& { return *reinterpret_cast<Raw*>(regs.data() + (size_t)tid * WORDS); };
   const uint32_t ntiles = cols / PL::W;
-  for (int tid = 0; tid < PL::NT; tid++) PP::load(tid, 0, in, cols, raw[tid].v);
+  for (int tid = 0; tid < PL::NT; tid++) PP::load(tid, 0, in, cols, raw(tid));
+  static_assert(PL::D == 3, "the emulated schedule has three digits");
+  constexpr int G1 = (PL::W << (PL::L - PL::R2)) / PL::NT;
+  constexpr int G2 = (PL::W << (PL::L - PL::RD)) / PL::NT;
+  constexpr int NP = 2 * G1 + G2;
   for (uint32_t tile = 0; tile < ntiles; tile++) {
-    for (int tid = 0; tid < PL::NT; tid++) PP::first(tid, raw[tid].v, lds.data(), T, inv1);
+    const bool more = tile + 1 < ntiles;
     for (int tid = 0; tid < PL::NT; tid++)
-      encode_step<PL, 1, true>(tid, tile, in, out, k, cols, 0, lds.data(), T, [&](int gi) {
-        if (gi < PP::G) PP::store_even(tid, gi, tile, out, cols, raw[tid].v[gi]);
-      });
-    if (tile + 1 < ntiles)
-      for (int tid = 0; tid < PL::NT; tid++) PP::load(tid, tile + 1, in, cols, raw[tid].v);
-    run_late_steps<PL, 2>(tile, in, out, cols, lds.data(), T);
+      PP::first(tid, raw(tid), lds.data(), T, inv1,
+                [&](int gi) { PP::store_even(tid, gi, tile, out, cols, raw(tid)[gi]); });
+    run_step_loading<PL, 1>(tile, in, out, cols, lds.data(), T, more, tile + 1, 0, NP, &regs, WORDS);
+    run_step_loading<PL, 2>(tile, in, out, cols, lds.data(), T, more, tile + 1, G1, NP, &regs, WORDS);
+    run_step_loading<PL, 3>(tile, in, out, cols, lds.data(), T, more, tile + 1, G1 + G2, NP, &regs, WORDS);
+    run_late_steps<PL, 4>(tile, in, out, cols, lds.data(), T);
   }
   return 0;
 }

@@ -125,10 +125,10 @@ def test_batched_plans_emulation(k, variant):
     assert np.array_equal(sim_encode(data, variant=variant), O.encode(data, threads=4))
 
 
-@pytest.mark.parametrize("k,cols", [(2048, 48)])
+@pytest.mark.parametrize("k,cols", [(2048, 48), (4096, 24)])
 def test_persistent_kernel_emulation(k, cols):
     # first pass from registers loaded while the previous tile was in flight,
-    # even-row stores spread over the second pass
+    # in parts over the middle passes; even-row stores during the first pass
     from helpers import sim_encode_persistent
     rng = np.random.default_rng(k)
     data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)

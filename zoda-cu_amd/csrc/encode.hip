@@ -47,18 +47,25 @@ __global__ __launch_bounds__(PL::NT) void encode_kernel(const uint32_t* __restri
   }
 }
 
-// Persistent form with the first pass pipelined through registers (Pipe):
-// the loads of the next tile are issued once the current tile's values have
-// left the registers (after its second pass) and complete behind the
-// remaining passes; the even-row stores are spread over the second pass.
+// Persistent form with the first pass pipelined through registers (Pipe).
+// Every tile moves 4 bytes in and 8 out per value and a work-group fills its
+// CU, so the memory work is spread over the passes rather than issued in
+// bursts the CU's queues cannot absorb (issuing the next tile's 32 loads per
+// thread at once stalls every wave until most of them have returned): the
+// even-row stores go out during the first pass, straight from the registers
+// it reads; the next tile's loads are issued in parts, one per group of the
+// middle passes; the odd-row stores leave the last pass as it produces them.
 template <class PL>
 __global__ __launch_bounds__(PL::NT) void encode_pipelined(const uint32_t* __restrict__ in,
                                                            uint32_t* __restrict__ out,
                                                            uint32_t cols, uint32_t ntiles,
                                                            EncodeTabs T) {
   using PP = Pipe<PL>;
+  // groups per thread of the passes between the first and the last
   constexpr int G1 = (PL::W << (PL::L - (PL::D == 2 ? PL::RD : PL::R2))) / PL::NT;
-  static_assert(G1 >= PP::G, "the second pass must have a group for every first-pass group");
+  constexpr int G2 = PL::D == 3 ? (PL::W << (PL::L - PL::RD)) / PL::NT : 0;
+  constexpr int G3 = PL::D == 3 ? G1 : 0;
+  constexpr int NP = G1 + G2 + G3;  // load parts
   extern __shared__ int32_t lds[];
   encode_load_tables<PL>((int)threadIdx.x, lds, T);
   const uint32_t* sb = reinterpret_cast<const uint32_t*>(lds + PL::TILE_WORDS);
@@ -71,20 +78,27 @@ __global__ __launch_bounds__(PL::NT) void encode_pipelined(const uint32_t* __res
   for (; l < ntiles; l += gridDim.x) {
     const uint32_t tile = xcd_tile(l, ntiles);
     const uint32_t next = l + gridDim.x;
+    const bool more = next < ntiles;
+    const uint32_t ntile = more ? xcd_tile(next, ntiles) : 0u;
     const int tid = (int)threadIdx.x;
-    PP::first(tid, raw, lds, T, inv1);
+    PP::first(tid, raw, lds, T, inv1, [&](int gi) { PP::store_even(tid, gi, tile, out, cols, raw[gi]); });
     __syncthreads();
     encode_step<PL, 1, true>(tid, tile, in, out, k, cols, 0, lds, T, [&](int gi) {
-      if (gi < PP::G) PP::store_even(tid, gi, tile, out, cols, raw[gi]);
+      if (more) PP::load_part(tid, ntile, in, cols, raw, gi, NP);
     });
-    if (next < ntiles) PP::load(tid, xcd_tile(next, ntiles), in, cols, raw);
     __syncthreads();
-    encode_step<PL, 2, true>(tid, tile, in, out, k, cols, 0, lds, T);
-    if constexpr (PL::STEPS > 3) {
+    if constexpr (PL::D == 3) {
+      encode_step<PL, 2, true>(tid, tile, in, out, k, cols, 0, lds, T, [&](int gi) {
+        if (more) PP::load_part(tid, ntile, in, cols, raw, G1 + gi, NP);
+      });
       __syncthreads();
-      encode_step<PL, 3, true>(tid, tile, in, out, k, cols, 0, lds, T);
+      encode_step<PL, 3, true>(tid, tile, in, out, k, cols, 0, lds, T, [&](int gi) {
+        if (more) PP::load_part(tid, ntile, in, cols, raw, G1 + G2 + gi, NP);
+      });
       __syncthreads();
       encode_step<PL, 4, true>(tid, tile, in, out, k, cols, 0, lds, T);
+    } else {
+      encode_step<PL, 2, true>(tid, tile, in, out, k, cols, 0, lds, T);
     }
     __syncthreads();  // the next tile's first pass writes LDS
   }
@@ -157,7 +171,8 @@ static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32
   const bool full = cols % PL::W == 0 && k == (uint32_t)PL::S;
   const bool row = layout == ZODA_LAYOUT_ROW;
   if constexpr (PL::PERSIST) {
-    if (full && row) {
+    // Pipe addresses rows within a digit with 32-bit byte offsets.
+    if (full && row && ((uint64_t)cols << (PL::LO1 + 1)) * 4 < (1ull << 32)) {
       const void* pfn = (const void*)encode_pipelined<PL>;
       uint32_t grid = 0;
       ZODA_TRY(kernel_setup(st, pfn, PL::NT, lds_bytes, true, &grid));

@@ -39,6 +39,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "field.hpp"
 #include "fp64.hpp"
 
@@ -352,48 +354,80 @@ ZHD void encode_step(int tid, uint32_t tile, const uint32_t* __restrict__ in,
 }
 
 // Register pipeline of the first pass for whole tiles in the row layout
-// (plans with D >= 2).  load() brings a tile's first-pass
-// values into registers -- issued while the previous tile is still being
-// transformed, so the loads are in flight behind that work -- first()
-// transforms them into the LDS tile, and store_even() writes them to the even
-// output rows, one group at a time during the next pass, so the stores too
-// are spread over compute instead of issued in one burst.
+// (plans with D >= 2).  load_part() brings part of a tile's first-pass values
+// into registers -- issued group by group while the previous tile is still
+// being transformed, so the loads are in flight behind that work and never
+// queue up in one burst -- first() transforms them into the LDS tile, and
+// store_even() writes them to the even output rows, one group at a time as
+// first() reaches it.
 template <class PL>
 struct Pipe {
   static constexpr int R = PL::R1, RR = 1 << R, LO = PL::LO1;
   static constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
   static_assert(PL::D >= 2 && (PL::W << (PL::L - R)) % PL::NT == 0, "pipeline shape");
 
+  // Byte offset of group gi's first value in a row-major array of `cols`
+  // columns (`stride` rows per top-digit row: 1 for the input, 2 for the even
+  // output rows).  Rows of the top digit are 2^LO apart, so one uniform base
+  // per digit value plus this 32-bit offset addresses every value; the caller
+  // guarantees (cols << (LO + 1)) * 4 < 2^32.
+  static ZHD uint32_t offset(int tid, int gi, uint32_t tile, uint32_t cols, uint32_t stride) {
+    const uint32_t g = (uint32_t)(gi * PL::NT + tid);
+    const uint32_t c = g % PL::W, u = g / PL::W;  // u < 2^LO: the row within the digit
+    return (u * stride * cols + tile * PL::W + c) * 4u;
+  }
+
+  template <class T>
+  static ZHD T* at(T* base, uint32_t byte_offset) {
+    return reinterpret_cast<T*>(reinterpret_cast<
+        typename std::conditional<std::is_const<T>::value, const char, char>::type*>(base) + byte_offset);
+  }
+
   static ZHD void load(int tid, uint32_t tile, const uint32_t* __restrict__ in, uint32_t cols,
                        uint32_t (&raw)[G][RR]) {
     const size_t istep = (size_t)cols << LO;
 #pragma unroll
     for (int gi = 0; gi < G; gi++) {
-      const uint32_t g = (uint32_t)(gi * PL::NT + tid);
-      const uint32_t c = g % PL::W, u = g / PL::W;
-      const uint32_t* src = in + (size_t)row_of<LO, R>(u, 0) * cols + (size_t)tile * PL::W + c;
+      const uint32_t off = offset(tid, gi, tile, cols, 1);
 #pragma unroll
-      for (int m = 0; m < RR; m++) raw[gi][m] = src[(size_t)m * istep];
+      for (int m = 0; m < RR; m++) raw[gi][m] = *at(in + (size_t)m * istep, off);
+    }
+  }
+
+  // Part p of np of load(): value q = gi * RR + m is loaded by part
+  // q * np / (G * RR), so np calls made from successive groups of later passes
+  // spread the loads over that work.  After unrolling every index is a
+  // constant and the conditions fold away.
+  static ZHD void load_part(int tid, uint32_t tile, const uint32_t* __restrict__ in, uint32_t cols,
+                            uint32_t (&raw)[G][RR], int p, int np) {
+    const size_t istep = (size_t)cols << LO;
+#pragma unroll
+    for (int gi = 0; gi < G; gi++) {
+      const uint32_t off = offset(tid, gi, tile, cols, 1);
+#pragma unroll
+      for (int m = 0; m < RR; m++)
+        if ((gi * RR + m) * np / (G * RR) == p) raw[gi][m] = *at(in + (size_t)m * istep, off);
     }
   }
 
   static ZHD void store_even(int tid, int gi, uint32_t tile, uint32_t* __restrict__ out,
                              uint32_t cols, const uint32_t (&r)[RR]) {
     const size_t ostep = (size_t)cols << (LO + 1);
-    const uint32_t g = (uint32_t)(gi * PL::NT + tid);
-    const uint32_t c = g % PL::W, u = g / PL::W;
-    uint32_t* dst = out + (size_t)(2 * row_of<LO, R>(u, 0)) * cols + (size_t)tile * PL::W + c;
+    const uint32_t off = offset(tid, gi, tile, cols, 2);
 #pragma unroll
-    for (int m = 0; m < RR; m++) dst[(size_t)m * ostep] = r[m];
+    for (int m = 0; m < RR; m++) *at(out + (size_t)m * ostep, off) = r[m];
   }
 
-  // The first pass (INTT of the top digit) from registers into the LDS tile.
+  // The first pass (INTT of the top digit) from registers into the LDS tile;
+  // hook(gi) is called before group gi is transformed.
+  template <class Hook = NoHook>
   static ZHD void first(int tid, const uint32_t (&raw)[G][RR], int32_t* __restrict__ lds,
-                        const EncodeTabs& T, const TwTab inv1) {
+                        const EncodeTabs& T, const TwTab inv1, const Hook& hook = Hook()) {
     const Tw* rinv = T.roots + 64;
 #pragma unroll
     for (int gi = 0; gi < G; gi++) {
       group_fence();
+      hook(gi);
       const uint32_t g = (uint32_t)(gi * PL::NT + tid);
       const uint32_t c = g % PL::W, u = g / PL::W;
       double x[RR];
