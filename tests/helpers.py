@@ -96,6 +96,8 @@ def sim_lib():
         lib.zoda_sim_sha256.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_void_p]
         lib.zoda_sim_rlc_coefficient.restype = None
         lib.zoda_sim_rlc_coefficient.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p]
+        lib.zoda_sim_decode.restype = ctypes.c_int
+        lib.zoda_sim_decode.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p]
         lib.zoda_sim_encode_persistent.restype = ctypes.c_int
         lib.zoda_sim_encode_persistent.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p]
         _sim = lib
@@ -124,4 +126,18 @@ def sim_encode(data: np.ndarray, n: int | None = None, layout: int = 0, variant:
     rc = sim_lib().zoda_sim_encode_variant(d.ctypes.data, k, cols, n, out.ctypes.data, layout, variant)
     if rc != 0:
         raise ValueError(f"shape not covered by the fused kernel (k={k}, n={n})")
+    return out
+
+
+def sim_decode(enc: np.ndarray, known_rows) -> np.ndarray:
+    """The decoder's FP64 column kernel run on the host (tests/sim): the k x
+    cols data from the 2k x cols extended square and its known rows."""
+    n, cols = enc.shape
+    k = n // 2
+    out = np.zeros((k, cols), dtype=np.uint32)
+    e = np.ascontiguousarray(enc, dtype=np.uint32)
+    rows = np.ascontiguousarray(known_rows, dtype=np.uint32)
+    rc = sim_lib().zoda_sim_decode(e.ctypes.data, k, cols, rows.ctypes.data, rows.size, out.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"no FP64 decoder plan for k={k} (status {rc})")
     return out

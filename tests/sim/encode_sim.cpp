@@ -13,6 +13,7 @@
 #include "../../zoda-cu_amd/csrc/encode_plans.hpp"
 #include "../../zoda-cu_amd/csrc/sha256.hpp"
 #include "../../zoda-cu_amd/csrc/encode_tables.hpp"
+#include "../../zoda-cu_amd/csrc/decode_core.hpp"
 
 using namespace zoda;
 
@@ -150,4 +151,80 @@ extern "C" void zoda_sim_sha256(const uint8_t* data, size_t len, uint8_t* out) {
 
 extern "C" void zoda_sim_rlc_coefficient(const uint8_t* root, uint32_t len, uint64_t col, uint32_t* out) {
   rlc_coefficient(root, len, col, out);
+}
+
+// ---- erasure decoder (decode_core.hpp) ------------------------------------------
+
+template <class PL, int STEP>
+static void dec_run_steps(int32_t* lds, const DecodeTabs& T, const uint32_t* zinv) {
+  for (int tid = 0; tid < PL::NT; tid++) dec_step<PL, STEP>(tid, lds, T, zinv);
+  if constexpr (STEP < 9) dec_run_steps<PL, STEP + 1>(lds, T, zinv);
+}
+
+// The decoder's column kernel on the host: `enc` is the 2k x cols extended
+// square (row-major), the first k distinct of `known_rows` are used, `out`
+// receives the k x cols data.  The vanishing-polynomial values are computed
+// here directly from their definition; the scratch holds the known rows by
+// rank, as the GPU's gather leaves it.  Returns -1 for sides without an FP64
+// plan.
+extern "C" int zoda_sim_decode(const uint32_t* enc, uint32_t k, uint32_t cols, const uint32_t* known_rows,
+                               uint32_t nknown, uint32_t* out) {
+  const uint32_t n = 2 * k;
+  uint32_t L = 0;
+  while ((1u << L) < n) L++;
+  if ((1u << L) != n) return -1;
+  std::vector<uint8_t> known(n, 0);
+  uint32_t have = 0;
+  for (uint32_t i = 0; i < nknown && have < k; i++)
+    if (known_rows[i] < n && !known[known_rows[i]]) {
+      known[known_rows[i]] = 1;
+      have++;
+    }
+  if (have < k) return -2;
+  const uint32_t w = root_of_unity(L), h = 31;
+  std::vector<uint32_t> x(n), erased;
+  for (uint32_t r = 0, cur = 1; r < n; r++, cur = mul_mod_host(cur, w)) {
+    x[r] = cur;
+    if (!known[r]) erased.push_back(cur);
+  }
+  auto Z = [&](uint32_t pt) {
+    uint32_t acc = 1;
+    for (uint32_t e : erased) acc = mul_mod_host(acc, (pt + P - e) % P);
+    return acc;
+  };
+  std::vector<uint32_t> zk(n, 0), zinv(n);
+  for (uint32_t r = 0; r < n; r++) {
+    if (known[r]) zk[r] = Z(x[r]);
+    zinv[r] = inv_mod_host(Z(mul_mod_host(h, x[r])));
+  }
+  // The rank-packed scratch: per column the k known rows (in increasing
+  // order); `rows` lists them, then the erased rows.
+  std::vector<uint32_t> rows;
+  for (uint32_t r = 0; r < n; r++)
+    if (known[r]) rows.push_back(r);
+  for (uint32_t r = 0; r < n; r++)
+    if (!known[r]) rows.push_back(r);
+  std::vector<uint32_t> T((size_t)cols * k);
+  for (uint32_t i = 0; i < k; i++)
+    for (uint32_t c = 0; c < cols; c++)
+      T[(size_t)c * k + i] = mul_mod_host(enc[(size_t)rows[i] * cols + c], zk[rows[i]]);
+  const int rc = with_decode_plan((int)L, [&](auto plan) -> int {
+    using PL = decltype(plan);
+    const DecodeTableImage im = build_decode_tables(PL::L, PL::R1, PL::R2, PL::RD, h);
+    static const DecodeConst uniform = make_decode_const(31);  // the kernel's constant-memory tables
+    const DecodeTabs tabs = make_decode_tabs(PL::L, im, im.packed.data(), uniform);
+    std::vector<int32_t> lds(PL::LDS_WORDS);
+    for (int tid = 0; tid < PL::NT; tid++) dec_load_tables<PL>(tid, lds.data(), tabs);
+    const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
+    for (uint32_t tile = 0; tile < ntiles; tile++) {
+      for (int tid = 0; tid < PL::NT; tid++) dec_load<PL>(tid, tile, cols, T.data(), rows.data(), lds.data());
+      dec_run_steps<PL, 0>(lds.data(), tabs, zinv.data());
+      for (int tid = 0; tid < PL::NT; tid++) dec_store<PL>(tid, tile, cols, T.data(), lds.data());
+    }
+    return 0;
+  });
+  if (rc != 0) return rc;
+  for (uint32_t j = 0; j < k; j++)
+    for (uint32_t c = 0; c < cols; c++) out[(size_t)j * cols + c] = T[(size_t)c * k + j];
+  return 0;
 }

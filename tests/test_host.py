@@ -135,6 +135,19 @@ def test_persistent_kernel_emulation(k, cols):
     assert np.array_equal(sim_encode_persistent(data), O.encode(data, threads=4))
 
 
+@pytest.mark.parametrize("k,cols", [(128, 9), (256, 5), (512, 6), (1024, 3), (2048, 2)])
+def test_decoder_kernel_emulation(k, cols):
+    # the FP64 column kernel of the erasure decoder, step by step on the host,
+    # from the odd rows, a random half, and the last k rows; ragged last tiles
+    from helpers import sim_decode
+    rng = np.random.default_rng(k + cols)
+    data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)
+    data[0, 0] = P - 1
+    enc = O.encode(data, threads=4)
+    for known in (np.arange(1, 2 * k, 2), rng.choice(2 * k, size=k, replace=False), np.arange(k, 2 * k)):
+        assert np.array_equal(sim_decode(enc, rng.permutation(known)), data)
+
+
 # ---- SHA-256, RLC coefficients and the data root (src/zoda/rlc.rs, utils.rs:51-59) ----
 
 def _sim_sha256(data: bytes) -> bytes:

@@ -59,11 +59,20 @@ struct EncodeTableSet {
   EncodeTabs* tabs = nullptr;  // host-side struct (owned)
 };
 
+struct DecodeTabs;
+// Device copy of the erasure decoder's LDS tables for one n and the
+// DecodeTabs pointing into it.
+struct DecodeTableSet {
+  uint32_t* packed = nullptr;
+  DecodeTabs* tabs = nullptr;  // host-side struct (owned)
+};
+
 struct DeviceState {
   int device = 0;
   std::recursive_mutex mu;
   std::map<uint32_t, MontTables> mont;           // standard roots, keyed by log n
   std::map<uint32_t, EncodeTableSet> encode;     // keyed by encode_table_key()
+  std::map<uint32_t, DecodeTableSet> decode;     // keyed by log n
   int batch_variant[16] = {0};                   // autotuned batched-plan variant per log side
   // Drop-in entry points: tables built from the caller's roots.
   std::map<uint32_t, std::vector<uint32_t>> user_roots;

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "decode_core.hpp"
 
 namespace zoda {
 
@@ -56,15 +57,17 @@ __device__ uint32_t mont_pow(uint32_t a, uint32_t e) {
 }
 
 // x_m[r] = w^r in Montgomery form; ep_m: the erased points, Montgomery form.
-// zk_m[r] = Z(w^r) (0 on erased rows); zinv_br_m[i] = 1 / Z(h w^(brev(i))).
+// zk_m[r] = Z(w^r) (0 on erased rows).  With `natural`, zinv[i] = 1 / Z(h w^i)
+// as a canonical value; otherwise zinv[i] = 1 / Z(h w^(brev(i))) in
+// Montgomery form.
 __global__ void dec_prep(const uint32_t* __restrict__ x_m, const uint8_t* __restrict__ known,
                          const uint32_t* __restrict__ ep_m, uint32_t ne, uint32_t log_n, uint32_t h_m,
-                         uint32_t* __restrict__ zk_m, uint32_t* __restrict__ zinv_br_m) {
+                         int natural, uint32_t* __restrict__ zk_m, uint32_t* __restrict__ zinv) {
   const uint32_t n = 1u << log_n;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 2 * n) return;
   const bool coset = t >= n;
-  const uint32_t r = coset ? brev_n(t - n, log_n) : t;
+  const uint32_t r = coset ? (natural ? t - n : brev_n(t - n, log_n)) : t;
   if (!coset && !known[r]) {
     zk_m[r] = 0;
     return;
@@ -72,13 +75,17 @@ __global__ void dec_prep(const uint32_t* __restrict__ x_m, const uint8_t* __rest
   const uint32_t x = coset ? mont_mul(h_m, x_m[r]) : x_m[r];
   uint32_t acc = MONT_ONE;
   for (uint32_t j = 0; j < ne; j++) acc = mont_mul(acc, sub_mod(x, ep_m[j]));
-  if (coset)
-    zinv_br_m[t - n] = mont_pow(acc, P - 2);
-  else
+  if (coset) {
+    const uint32_t inv = mont_pow(acc, P - 2);
+    zinv[t - n] = natural ? mont_mul(inv, 1u) : inv;  // mont_mul by 1 leaves Montgomery form
+  } else {
     zk_m[r] = acc;
+  }
 }
 
-// T[c * n + r] = enc[r * cols + c] * Z(w^r) (a 32 x 32 tile per block).
+// T[c * n + r] = enc[r * cols + c] * Z(w^r) (a 32 x 32 tile per block).  The
+// erased rows are never read and are written as 0 (whole 128-byte segments
+// write faster than the known rows alone).
 __global__ void dec_gather(const uint32_t* __restrict__ enc, uint32_t cols, uint32_t n,
                            const uint32_t* __restrict__ zk_m, uint32_t* __restrict__ T) {
   __shared__ uint32_t tile[TT][TT + 1];
@@ -97,6 +104,31 @@ __global__ void dec_gather(const uint32_t* __restrict__ enc, uint32_t cols, uint
   for (uint32_t i = ty; i < TT; i += 8) {
     const uint32_t c = c0 + i, r = r0 + tx;
     if (c < cols && r < n) T[(size_t)c * n + r] = tile[tx][i];
+  }
+}
+
+// The known rows only, packed by rank: T[c * k + i] = enc[rows[i] * cols + c]
+// * Z(w^rows[i]) for the k known rows rows[0..k) (a 32 x 32 tile of (rank,
+// column) per block: whole 128-byte segments both ways).
+__global__ void dec_gather_ranked(const uint32_t* __restrict__ enc, uint32_t cols, uint32_t k,
+                                  const uint32_t* __restrict__ rows, const uint32_t* __restrict__ zk_m,
+                                  uint32_t* __restrict__ T) {
+  __shared__ uint32_t tile[TT][TT + 1];
+  const uint32_t c0 = blockIdx.x * TT, i0 = blockIdx.y * TT;
+  const uint32_t tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+  for (uint32_t q = ty; q < TT; q += 8) {
+    const uint32_t i = i0 + q, c = c0 + tx;
+    uint32_t v = 0;
+    if (i < k && c < cols) {
+      const uint32_t r = rows[i];
+      v = mont_mul(enc[(size_t)r * cols + c], zk_m[r]);
+    }
+    tile[q][tx] = v;
+  }
+  __syncthreads();
+  for (uint32_t q = ty; q < TT; q += 8) {
+    const uint32_t c = c0 + q, i = i0 + tx;
+    if (c < cols && i < k) T[(size_t)c * k + i] = tile[tx][q];
   }
 }
 
@@ -160,6 +192,99 @@ __global__ __launch_bounds__(DEC_NT) void dec_fused(uint32_t* __restrict__ T, ui
   for (uint32_t j = threadIdx.x; j < k; j += DEC_NT) col[j] = v[brev_n(j, log_n - 1)];
 }
 
+__constant__ DecodeConst kDecodeConst = make_decode_const(DEC_COSET);
+
+// Persistent form for 2^8 <= n <= 2^12: tiles of W columns of the
+// rank-packed scratch (k values each) through both FP64 transform pairs in
+// LDS (decode_core.hpp); the k data values of each column replace its entries.
+template <class PL>
+__global__ __launch_bounds__(PL::NT, 4) void dec_columns(uint32_t* __restrict__ T, uint32_t cols, uint32_t ntiles,
+                                                      const uint32_t* __restrict__ rows,
+                                                      const uint32_t* __restrict__ zinv, DecodeTabs tabs) {
+  extern __shared__ int32_t lds[];
+  dec_load_tables<PL>((int)threadIdx.x, lds, tabs);
+  DecodeTabs ct = tabs;  // the uniform tables from constant memory
+  ct.roots = kDecodeConst.roots;
+  ct.mid[0] = kDecodeConst.mid[PL::L - DEC_FP64_MIN_LOG][0];
+  ct.mid[1] = kDecodeConst.mid[PL::L - DEC_FP64_MIN_LOG][1];
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // A thread's addresses are the same for every tile.  Making its index
+    // depend on the tile (through an asm the optimizer cannot see through,
+    // not volatile: no side effects) keeps that arithmetic in the loop
+    // instead of hoisted out of it and held -- or spilled -- for the whole
+    // kernel.
+    int tid = (int)threadIdx.x;
+    asm("" : "+v"(tid) : "s"(tile));
+    dec_load<PL>(tid, tile, cols, T, rows, lds);
+    __syncthreads();  // (on the first tile also the tables)
+    dec_step<PL, 0>(tid, lds, ct, zinv);
+    __syncthreads();
+    dec_step<PL, 1>(tid, lds, ct, zinv);
+    __syncthreads();
+    dec_step<PL, 2>(tid, lds, ct, zinv);
+    __syncthreads();
+    dec_step<PL, 3>(tid, lds, ct, zinv);
+    __syncthreads();
+    dec_step<PL, 4>(tid, lds, ct, zinv);
+    __syncthreads();
+    dec_step<PL, 5>(tid, lds, ct, zinv);
+    __syncthreads();
+    dec_step<PL, 6>(tid, lds, ct, zinv);
+    __syncthreads();
+    dec_step<PL, 7>(tid, lds, ct, zinv);
+    __syncthreads();
+    dec_step<PL, 8>(tid, lds, ct, zinv);
+    __syncthreads();
+    dec_step<PL, 9>(tid, lds, ct, zinv);
+    __syncthreads();
+    dec_store<PL>(tid, tile, cols, T, lds);
+    __syncthreads();  // the next tile's load overwrites LDS
+  }
+}
+
+// Tables of the FP64 decoder for n = 2^L, built and uploaded once per device.
+int get_decode_tables(DeviceState* st, int L, int r1, int r2, int rd, const DecodeTabs** out) {
+  std::lock_guard<std::recursive_mutex> lock(st->mu);
+  auto it = st->decode.find((uint32_t)L);
+  if (it == st->decode.end()) {
+    const DecodeTableImage im = build_decode_tables(L, r1, r2, rd, DEC_COSET);
+    DecodeTableSet set;
+    if (hipMalloc((void**)&set.packed, im.packed.size() * sizeof(uint32_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(ZODA_ENOMEM, "decoder table allocation failed");
+    }
+    const hipError_t e =
+        hipMemcpy(set.packed, im.packed.data(), im.packed.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(set.packed);
+      return fail((int)e, "copying decoder tables failed: %s", hipGetErrorString(e));
+    }
+    set.tabs = new DecodeTabs(make_decode_tabs(L, im, set.packed, DecodeConst{}));
+    // The kernel points these at its constant-memory copy (kDecodeConst).
+    set.tabs->roots = nullptr;
+    set.tabs->mid[0] = set.tabs->mid[1] = nullptr;
+    it = st->decode.emplace((uint32_t)L, set).first;
+  }
+  *out = it->second.tabs;
+  return 0;
+}
+
+// Once per device: the persistent grid of dec_columns<PL> (every CU filled).
+template <class PL>
+int decode_grid(DeviceState* st, uint32_t* grid) {
+  const void* fn = (const void*)dec_columns<PL>;
+  std::lock_guard<std::recursive_mutex> lock(st->mu);
+  auto it = st->kernel_setup.find(fn);
+  if (it == st->kernel_setup.end()) {
+    int per_cu = 0, cus = 0;
+    ZODA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, PL::NT, PL::LDS_WORDS * sizeof(int32_t)));
+    ZODA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, st->device));
+    it = st->kernel_setup.emplace(fn, (uint32_t)std::max(1, per_cu * cus)).first;
+  }
+  *grid = it->second;
+  return 0;
+}
+
 // data[j * cols + c] = T[c * stride + j] for j < k.
 __global__ void dec_scatter(const uint32_t* __restrict__ T, uint32_t stride, uint32_t k, uint32_t cols,
                             uint32_t* __restrict__ data) {
@@ -202,9 +327,21 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
   if (have < k) return fail(ZODA_EINVAL, "%u distinct known rows, %u needed", have, k);
   DeviceState* st = nullptr;
   ZODA_TRY(current_state(&st));
+  // 2^8 <= n <= 2^12: the FP64 column kernel; other sizes: the Montgomery one.
+  const bool fp64 = (int)log_n >= DEC_FP64_MIN_LOG && (int)log_n <= DEC_FP64_MAX_LOG;
   MontTables tn, tk;
-  ZODA_TRY(get_mont_tables(st, log_n, &tn));
-  ZODA_TRY(get_mont_tables(st, log_n - 1, &tk));
+  const DecodeTabs* dtabs = nullptr;
+  uint32_t dgrid = 0;
+  if (fp64) {
+    ZODA_TRY(with_decode_plan((int)log_n, [&](auto plan) -> int {
+      using PL = decltype(plan);
+      ZODA_TRY(get_decode_tables(st, PL::L, PL::R1, PL::R2, PL::RD, &dtabs));
+      return decode_grid<PL>(st, &dgrid);
+    }));
+  } else {
+    ZODA_TRY(get_mont_tables(st, log_n, &tn));
+    ZODA_TRY(get_mont_tables(st, log_n - 1, &tk));
+  }
 
   // O(n) host vectors, Montgomery form: the points w^r, the erased points,
   // h^i / n and h^-i / n.
@@ -222,10 +359,16 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
     hq = mul_mod_host(hq, h_inv);
   }
   const uint32_t ne = (uint32_t)ep_m.size();  // = k
+  std::vector<uint32_t> rows;  // the known rows, then the erased ones, each increasing
+  rows.reserve(n);
+  for (uint32_t r = 0; r < n; r++)
+    if (known[r]) rows.push_back(r);
+  for (uint32_t r = 0; r < n; r++)
+    if (!known[r]) rows.push_back(r);
 
-  // One device block: x_m | hpow_m | hinv_m | ep_m | zk_m | zinv_br_m | known.
+  // One device block: x_m | hpow_m | hinv_m | ep_m | rows | zk_m | zinv | known.
   hipStream_t s = (hipStream_t)stream;
-  const size_t up_words = (size_t)n + n + k + ne;
+  const size_t up_words = (size_t)n + n + k + ne + n;
   const size_t words = up_words + n + n;
   uint32_t* d_blk = nullptr;
   uint32_t* T = nullptr;
@@ -234,7 +377,8 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
   uint32_t* d_hpow = d_x + n;
   uint32_t* d_hinv = d_hpow + n;
   uint32_t* d_ep = d_hinv + k;
-  uint32_t* d_zk = d_ep + ne;
+  uint32_t* d_rows = d_ep + ne;
+  uint32_t* d_zk = d_rows + n;
   uint32_t* d_zinv = d_zk + n;
   uint8_t* d_known = reinterpret_cast<uint8_t*>(d_zinv + n);
   std::vector<uint32_t> host;
@@ -243,27 +387,42 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
   host.insert(host.end(), hpow_m.begin(), hpow_m.end());
   host.insert(host.end(), hinv_m.begin(), hinv_m.end());
   host.insert(host.end(), ep_m.begin(), ep_m.end());
+  host.insert(host.end(), rows.begin(), rows.end());
   int rc = 0;
   if (hipMemcpy(d_blk, host.data(), up_words * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(d_known, known.data(), n, hipMemcpyHostToDevice) != hipSuccess)
     rc = fail(ZODA_ESTATE, "decode vectors upload failed");
-  if (!rc && hipMallocAsync((void**)&T, (size_t)cols * n * sizeof(uint32_t), s) != hipSuccess) {
+  // Scratch: the k known rows per column (FP64 path) or all n rows.
+  const uint32_t t_rows = fp64 ? k : n;
+  if (!rc && hipMallocAsync((void**)&T, (size_t)cols * t_rows * sizeof(uint32_t), s) != hipSuccess) {
     T = nullptr;
     rc = fail(ZODA_ENOMEM, "decode scratch allocation failed");
   }
   const size_t lds = (size_t)n * sizeof(uint32_t);
-  if (!rc && lds > 65536 &&
+  if (!rc && !fp64 && lds > 65536 &&
       hipFuncSetAttribute((const void*)dec_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
     rc = fail(ZODA_ESTATE, "raising the decoder's LDS limit failed");
   if (!rc) {
     hipLaunchKernelGGL(dec_prep, dim3((2 * n + 255) / 256), dim3(256), 0, s, d_x, d_known, d_ep, ne, log_n,
-                       to_mont_host(DEC_COSET), d_zk, d_zinv);
-    hipLaunchKernelGGL(dec_gather, dim3((cols + TT - 1) / TT, (n + TT - 1) / TT), dim3(TT, 8), 0, s, d_enc, cols,
-                       n, d_zk, T);
-    hipLaunchKernelGGL(dec_fused, dim3(cols), dim3(DEC_NT), lds, s, T, log_n, tn.fwd, tn.inv,
-                       tk.fwd, d_hpow, d_zinv, d_hinv);
-    hipLaunchKernelGGL(dec_scatter, dim3((k + TT - 1) / TT, (cols + TT - 1) / TT), dim3(TT, 8), 0, s, T, n, k, cols,
-                       d_data);
+                       to_mont_host(DEC_COSET), fp64 ? 1 : 0, d_zk, d_zinv);
+    if (fp64) {
+      hipLaunchKernelGGL(dec_gather_ranked, dim3((cols + TT - 1) / TT, (k + TT - 1) / TT), dim3(TT, 8), 0, s, d_enc,
+                         cols, k, d_rows, d_zk, T);
+      (void)with_decode_plan((int)log_n, [&](auto plan) -> int {
+        using PL = decltype(plan);
+        const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
+        hipLaunchKernelGGL(dec_columns<PL>, dim3(std::min(dgrid, ntiles)), dim3(PL::NT),
+                           PL::LDS_WORDS * sizeof(int32_t), s, T, cols, ntiles, d_rows, d_zinv, *dtabs);
+        return 0;
+      });
+    } else {
+      hipLaunchKernelGGL(dec_gather, dim3((cols + TT - 1) / TT, (n + TT - 1) / TT), dim3(TT, 8), 0, s, d_enc, cols,
+                         n, d_zk, T);
+      hipLaunchKernelGGL(dec_fused, dim3(cols), dim3(DEC_NT), lds, s, T, log_n, tn.fwd, tn.inv, tk.fwd, d_hpow,
+                         d_zinv, d_hinv);
+    }
+    hipLaunchKernelGGL(dec_scatter, dim3((k + TT - 1) / TT, (cols + TT - 1) / TT), dim3(TT, 8), 0, s, T, t_rows, k,
+                       cols, d_data);
     if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "decode launch failed");
   }
   if (T && hipFreeAsync(T, s) != hipSuccess && !rc) rc = fail(ZODA_ESTATE, "hipFreeAsync failed");

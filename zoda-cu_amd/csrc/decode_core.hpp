@@ -1,0 +1,388 @@
+// Per-thread work of the erasure decoder's column transforms, shared by the
+// gfx950 kernel (decode.hip) and a CPU emulation used by the tests.
+//
+// A work-group holds W columns of n = 2^L values (y_r = out[r] * Z(w^r) on
+// the known rows, 0 on the erased ones) in LDS as int32.  Both halves of the decode are the transform pair
+//
+//     NTT_n( t (.) INTT'_n(x) ),   t_f = c * g^f * [f < lim]
+//
+// (INTT' without its 1/n, folded into c):
+//   first pair:  t_f = h^f / n                    -> g(h w^r) at row r
+//   scaling:     * 1/Z(h w^r)                     -> f(h w^r)
+//   second pair: t_f = h^-f / n for f < n/2, 0    -> f(w^r); the even rows are the data
+//
+// Each pair runs as five passes over three digits of the row index -- R1
+// (top), R2 and RD (bottom) bits -- in the arrangement of the fused encode
+// (encode_core.hpp): INTT passes on R1 and R2, a middle pass on RD (inverse
+// DFT, twist, forward DFT), NTT passes on R2 and R1.  The INTT passes leave
+// their outputs in digit-reversed order, so the frequency index of the value
+// at a middle-pass position is f = k1 + 2^R1 k2 + 2^(R1+R2) m.  The twist
+// factor c g^(k1 + 2^R1 k2) is constant over a middle-pass group and is
+// multiplied in by the second INTT pass, next to its twiddle; g^(2^(R1+R2) m)
+// [f < lim] comes from a table of 2^RD entries (lim = n/2 is m < 2^(RD-1)).
+//
+// Bounds: LDS holds |v| < 2^31 (canonical inputs, then values reduced below
+// 2^30 by the constant multiplication that ends every pass); a register DFT
+// of at most 32 points keeps them below 2^36, inside mulmod's 2^45.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "encode_core.hpp"
+#include "encode_tables.hpp"
+
+namespace zoda {
+
+struct DecodeTabs {
+  // canonical u32, copied into LDS: inv1 | fwd1 | fwd2 | inv2 | fold[0] | fold[1]
+  const uint32_t* packed;
+  uint32_t o_inv1, o_fwd1, o_fwd2, o_inv2, o_fold[2], packed_words;
+  // uniform tables (DecodeConst below; constant memory on the GPU)
+  const Tw* mid[2];  // per-value twist factors of each pair's middle pass
+  const Tw* roots;   // w_64^e for e < 64, then w_64^-e
+};
+
+template <int L_, int W_, int NT_, int R1_, int R2_, int RD_>
+struct DecodePlan {
+  static constexpr int L = L_, W = W_, NT = NT_, R1 = R1_, R2 = R2_, RD = RD_;
+  static_assert(R1 > 0 && R2 > 0 && RD > 0 && R1 <= 5 && R2 <= 5 && RD <= 5, "three register digits");
+  static_assert(R1 + R2 + RD == L, "digits must cover the row index");
+  static constexpr int S = 1 << L;  // n
+  static constexpr int LO1 = RD + R2, LO2 = RD;
+  // LDS word of (row, column): one pad word every 2^RD rows, so the middle
+  // pass's lanes (2^RD rows apart) fall into different banks.
+  static constexpr int TILE_VALUES = W * S;
+  static constexpr int TILE_WORDS = W * S + (S >> RD);
+  // inv1, fwd1 (n each), fwd2, inv2 (2^(LO2+R2) each), fold[2] (2^(R1+R2) each)
+  static constexpr int TW_WORDS = 2 * S + 2 * (1 << (LO2 + R2)) + 2 * (1 << (R1 + R2));
+  static constexpr int LDS_WORDS = TILE_WORDS + TW_WORDS;
+};
+
+template <class PL>
+ZHD uint32_t lds_at(uint32_t row, uint32_t c) {
+  return row * PL::W + c + (row >> PL::RD);
+}
+
+// INTT pass on the digit of R bits at bit LO: inverse DFT over the digit,
+// then the inter-digit twiddle from the LDS table `tl` ([k << LO | low]:
+// neighbouring lanes read neighbouring words) and, in the second pass
+// (FOLDED), the twist factor of the middle group from the LDS table `tf`
+// ([high << R | k]).
+template <class PL, int LO, int R, bool FOLDED>
+ZHD void dec_intt(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ tl,
+                  const uint32_t* __restrict__ tf, const Tw* __restrict__ roots) {
+  constexpr int RR = 1 << R;
+  constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
+  static_assert((PL::W << (PL::L - R)) % PL::NT == 0, "groups must divide evenly over the threads");
+  const TwTab own{nullptr, tl}, fold{nullptr, tf};
+#pragma unroll
+  for (int gi = 0; gi < G; gi++) {
+    group_fence();
+    const uint32_t g = (uint32_t)(gi * PL::NT + tid);
+    const uint32_t c = g % PL::W, u = g / PL::W;
+    double x[RR];
+#pragma unroll
+    for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<LO, R>(u, m), c)];
+    dft_reg<R>(x, roots + 64);
+    const uint32_t low = u & ((1u << LO) - 1u), high = (u >> LO) << R;
+    if constexpr (FOLDED) {
+#pragma unroll
+      for (int k = 0; k < RR; k++)
+        x[k] = mulmod(mulmod(x[k], tw_get<true>(own, ((uint32_t)k << LO) | low)), tw_get<true>(fold, high + k));
+    } else {
+      static_assert(FOLDED || LO + R == PL::L, "the unfolded pass is the top digit's");
+      x[0] = reduce(x[0]);  // twiddle w^0 = 1
+#pragma unroll
+      for (int k = 1; k < RR; k++) x[k] = mulmod(x[k], tw_get<true>(own, ((uint32_t)k << LO) | low));
+    }
+#pragma unroll
+    for (int m = 0; m < RR; m++) lds[lds_at<PL>(row_of<LO, R>(u, m), c)] = (int32_t)x[m];
+  }
+}
+
+// Middle pass on the bottom digit: inverse DFT, the per-value twist factor,
+// forward DFT, forward twiddle of the digit above (LDS table `ta`).
+template <class PL>
+ZHD void dec_mid(int tid, int32_t* __restrict__ lds, const Tw* __restrict__ mid, const uint32_t* __restrict__ ta,
+                 const Tw* __restrict__ roots) {
+  constexpr int R = PL::RD, RR = 1 << R, LOA = PL::LO2, RA = PL::R2;
+  constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
+  static_assert((PL::W << (PL::L - R)) % PL::NT == 0, "groups must divide evenly over the threads");
+  const TwTab above{nullptr, ta};
+#pragma unroll
+  for (int gi = 0; gi < G; gi++) {
+    group_fence();
+    const uint32_t g = (uint32_t)(gi * PL::NT + tid);
+    const uint32_t c = g % PL::W, u = g / PL::W;
+    double x[RR];
+#pragma unroll
+    for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<0, R>(u, m), c)];
+    dft_reg<R>(x, roots + 64);
+#pragma unroll
+    for (int m = 1; m < RR; m++) x[m] = mulmod(x[m], mid[m]);  // mid[0] = 1
+    dft_reg<R>(x, roots);
+#pragma unroll
+    for (int m = 0; m < RR; m++) {
+      const uint32_t j = row_of<0, R>(u, m);
+      const uint32_t k = (j >> LOA) & ((1u << RA) - 1u);
+      x[m] = mulmod(x[m], tw_get<true>(above, (k << LOA) | (j & ((1u << LOA) - 1u))));
+    }
+#pragma unroll
+    for (int m = 0; m < RR; m++) lds[lds_at<PL>(row_of<0, R>(u, m), c)] = (int32_t)x[m];
+  }
+}
+
+// NTT pass on the second digit: forward DFT, forward twiddle of the top digit.
+template <class PL>
+ZHD void dec_ntt2(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ ta, const Tw* __restrict__ roots) {
+  constexpr int R = PL::R2, RR = 1 << R, LO = PL::LO2, LOA = PL::LO1, RA = PL::R1;
+  constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
+  static_assert((PL::W << (PL::L - R)) % PL::NT == 0, "groups must divide evenly over the threads");
+  const TwTab above{nullptr, ta};
+#pragma unroll
+  for (int gi = 0; gi < G; gi++) {
+    group_fence();
+    const uint32_t g = (uint32_t)(gi * PL::NT + tid);
+    const uint32_t c = g % PL::W, u = g / PL::W;
+    double x[RR];
+#pragma unroll
+    for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<LO, R>(u, m), c)];
+    dft_reg<R>(x, roots);
+#pragma unroll
+    for (int m = 0; m < RR; m++) {
+      const uint32_t j = row_of<LO, R>(u, m);
+      const uint32_t k = (j >> LOA) & ((1u << RA) - 1u);
+      x[m] = mulmod(x[m], tw_get<true>(above, (k << LOA) | (j & ((1u << LOA) - 1u))));
+    }
+#pragma unroll
+    for (int m = 0; m < RR; m++) lds[lds_at<PL>(row_of<LO, R>(u, m), c)] = (int32_t)x[m];
+  }
+}
+
+// Last NTT pass (top digit), natural order out.  First pair (!DATA): every
+// row r times zinv[r] = 1/Z(h w^r) (canonical u32).  Second pair (DATA): only
+// the groups of even rows (the row's low bit is the group's), reduced to
+// canonical u32.
+template <class PL, bool DATA>
+ZHD void dec_ntt1(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ zinv, const Tw* __restrict__ roots) {
+  constexpr int R = PL::R1, RR = 1 << R, LO = PL::LO1;
+  constexpr int NG = (PL::W << (PL::L - R)) >> (DATA ? 1 : 0);
+  static_assert(NG % PL::NT == 0 || PL::NT % NG == 0, "groups must divide evenly over the threads");
+  constexpr int G = NG >= PL::NT ? NG / PL::NT : 1;
+  const TwTab zt{nullptr, zinv};
+#pragma unroll
+  for (int gi = 0; gi < G; gi++) {
+    group_fence();
+    const uint32_t g = (uint32_t)(gi * PL::NT + tid);
+    if (NG < PL::NT && g >= (uint32_t)NG) break;  // fewer groups than threads
+    const uint32_t c = g % PL::W, u = (g / PL::W) << (DATA ? 1 : 0);
+    double x[RR];
+#pragma unroll
+    for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<LO, R>(u, m), c)];
+    dft_reg<R>(x, roots);
+#pragma unroll
+    for (int m = 0; m < RR; m++) {
+      const uint32_t row = row_of<LO, R>(u, m);
+      if constexpr (DATA)
+        lds[lds_at<PL>(row, c)] = (int32_t)canonical(x[m]);
+      else
+        lds[lds_at<PL>(row, c)] = (int32_t)mulmod(x[m], tw_get<true>(zt, row));
+    }
+  }
+}
+
+// Tile `tile` of the scratch square into LDS.  The scratch holds, per column,
+// only the k = n/2 known rows, by rank: entry c * k + i is y at row rows[i]
+// (the known rows in increasing order); rows[k + i] are the erased rows,
+// which get 0.  Columns past `cols` (a ragged last tile) are zero.
+template <class PL>
+ZHD void dec_load(int tid, uint32_t tile, uint32_t cols, const uint32_t* __restrict__ T,
+                  const uint32_t* __restrict__ rows, int32_t* __restrict__ lds) {
+  constexpr uint32_t K = PL::S / 2;
+  const uint32_t* src = T + (size_t)tile * PL::W * K;
+  const uint32_t valid = (cols - tile * PL::W < (uint32_t)PL::W ? cols - tile * PL::W : (uint32_t)PL::W) * K;
+  for (uint32_t i = (uint32_t)tid; i < (uint32_t)PL::TILE_VALUES; i += PL::NT) {
+    const uint32_t c = i / (uint32_t)PL::S, q = i % (uint32_t)PL::S;  // q < K: known, else erased
+    const uint32_t r = rows[q];
+    const uint32_t idx = c * K + q;
+    lds[lds_at<PL>(r, c)] = (q < K && idx < valid) ? (int32_t)src[idx] : 0;
+  }
+}
+
+// The data rows (the even rows) of tile `tile` from LDS into the k entries of
+// each scratch column.
+template <class PL>
+ZHD void dec_store(int tid, uint32_t tile, uint32_t cols, uint32_t* __restrict__ T, const int32_t* __restrict__ lds) {
+  constexpr uint32_t K = PL::S / 2;
+  uint32_t* dst = T + (size_t)tile * PL::W * K;
+  const uint32_t wv = cols - tile * PL::W < (uint32_t)PL::W ? cols - tile * PL::W : (uint32_t)PL::W;
+  for (uint32_t i = (uint32_t)tid; i < wv * K; i += PL::NT) {
+    const uint32_t c = i / K, j = i % K;
+    dst[i] = (uint32_t)lds[lds_at<PL>(2 * j, c)];
+  }
+}
+
+// The packed LDS tables behind the tile.
+template <class PL>
+ZHD void dec_load_tables(int tid, int32_t* __restrict__ lds, const DecodeTabs& T) {
+  uint32_t* dst = reinterpret_cast<uint32_t*>(lds + PL::TILE_WORDS);
+  for (uint32_t i = (uint32_t)tid; i < T.packed_words; i += PL::NT) dst[i] = T.packed[i];
+}
+
+// Step STEP (0 .. 9) of both transform pairs on the tile in LDS; steps are
+// separated by a work-group barrier on the GPU.
+template <class PL, int STEP>
+ZHD void dec_step(int tid, int32_t* __restrict__ lds, const DecodeTabs& T, const uint32_t* __restrict__ zinv) {
+  const uint32_t* sb = reinterpret_cast<const uint32_t*>(lds + PL::TILE_WORDS);
+  constexpr int P = STEP / 5, K = STEP % 5;
+  if constexpr (K == 0)
+    dec_intt<PL, PL::LO1, PL::R1, false>(tid, lds, sb + T.o_inv1, nullptr, T.roots);
+  else if constexpr (K == 1)
+    dec_intt<PL, PL::LO2, PL::R2, true>(tid, lds, sb + T.o_inv2, sb + T.o_fold[P], T.roots);
+  else if constexpr (K == 2)
+    dec_mid<PL>(tid, lds, T.mid[P], sb + T.o_fwd2, T.roots);
+  else if constexpr (K == 3)
+    dec_ntt2<PL>(tid, lds, sb + T.o_fwd1, T.roots);
+  else
+    dec_ntt1<PL, P == 1>(tid, lds, zinv, T.roots);
+}
+
+// ---- host construction of the tables ------------------------------------------
+
+// Twist factor of the second INTT pass of a pair (digit of r bits, hb bits
+// of the top digit above it): entry [high << r | k] = c * g^(high + 2^hb k).
+inline void push_fold_table(std::vector<Tw>& v, int hb, int r, uint32_t g, uint32_t c) {
+  for (uint32_t high = 0; high < (1u << hb); high++)
+    for (uint32_t k = 0; k < (1u << r); k++)
+      v.push_back(make_tw(mul_mod_host(c, pow_mod_host(g, high + ((uint64_t)k << hb)))));
+}
+
+// Host image of the decoder's LDS tables for n = 2^L (canonical u32).
+struct DecodeTableImage {
+  std::vector<uint32_t> packed;
+  uint32_t o_inv1 = 0, o_fwd1 = 0, o_fwd2 = 0, o_inv2 = 0, o_fold[2] = {0, 0};
+};
+
+// Inverse twiddles of the digit of r bits at bit lo in [k << lo | low]
+// order: w^-(low k), w the root of order 2^(lo+r).
+inline void push_inverse_digit(std::vector<Tw>& v, int lo, int r) {
+  const uint32_t w = inv_mod_host(root_of_unity((uint32_t)(lo + r)));
+  for (uint32_t k = 0; k < (1u << r); k++) {
+    const uint32_t step = pow_mod_host(w, k);
+    uint32_t cur = 1;
+    for (uint32_t low = 0; low < (1u << lo); low++) {
+      v.push_back(make_tw(cur));
+      cur = mul_mod_host(cur, step);
+    }
+  }
+}
+
+inline DecodeTableImage build_decode_tables(int L, int r1, int r2, int rd, uint32_t h) {
+  DecodeTableImage im;
+  const int lo1 = rd + r2, lo2 = rd;
+  std::vector<Tw> t;
+  push_inverse_digit(t, lo1, r1);
+  im.o_inv1 = 0;
+  im.o_fwd1 = (uint32_t)t.size();
+  push_digit_table(t, lo1, r1, false);
+  im.o_fwd2 = (uint32_t)t.size();
+  push_digit_table(t, lo2, r2, false);
+  im.o_inv2 = (uint32_t)t.size();
+  push_inverse_digit(t, lo2, r2);
+  const uint32_t n_inv = inv_mod_host(1u << L);
+  const uint32_t g[2] = {h % P, inv_mod_host(h)};
+  for (int p = 0; p < 2; p++) {
+    im.o_fold[p] = (uint32_t)t.size();
+    push_fold_table(t, r1, r2, g[p], n_inv);
+  }
+  for (const Tw& x : t) im.packed.push_back((uint32_t)x.w);
+  return im;
+}
+
+// ---- compile-time constant tables (the kernel reads them from constant memory) --
+
+struct DecodeConst {
+  Tw roots[128];     // w_64^e for e < 64, then w_64^-e
+  Tw mid[5][2][32];  // per plan (L = 8..12) and pair: the middle pass's twist factors
+};
+
+ZHC Tw tw_const(uint32_t w) { return Tw{(double)w, (double)w / PD}; }
+
+// Middle-pass twist factors of pair p for n = 2^L with digits (r1, r2, rd):
+// g^(2^(r1+r2) m) for m below the pair's limit, 0 above (g = h, then 1/h).
+ZHC void decode_mid(Tw* out, int r1, int r2, int rd, uint32_t h, int p) {
+  const uint32_t g = p == 0 ? h % P : pow_mod_host(h, P - 2);
+  const uint32_t step = pow_mod_host(g, 1ull << (r1 + r2));
+  const uint32_t mlim = p == 0 ? 1u << rd : 1u << (rd - 1);
+  uint32_t cur = 1;
+  for (uint32_t m = 0; m < 32; m++) {
+    out[m] = tw_const(m < (1u << rd) && m < mlim ? cur : 0u);
+    cur = mul_mod_host(cur, step);
+  }
+}
+
+// Decoder plans: n = 2^L for 8 <= L <= 12, 512 threads, W columns so that
+// every pass has at least one group per thread (two work-groups per CU).
+//                      L   W   NT  R1 R2 RD
+using DecPlan8 = DecodePlan<8, 16, 512, 3, 3, 2>;
+using DecPlan9 = DecodePlan<9, 8, 512, 3, 3, 3>;
+using DecPlan10 = DecodePlan<10, 8, 512, 4, 3, 3>;
+using DecPlan11 = DecodePlan<11, 4, 512, 4, 4, 3>;
+using DecPlan12 = DecodePlan<12, 2, 512, 4, 4, 4>;
+constexpr int DEC_FP64_MIN_LOG = 8, DEC_FP64_MAX_LOG = 12;
+
+template <class PL>
+ZHC void decode_mid_plan(DecodeConst& c, uint32_t h) {
+  for (int p = 0; p < 2; p++) decode_mid(c.mid[PL::L - DEC_FP64_MIN_LOG][p], PL::R1, PL::R2, PL::RD, h, p);
+}
+
+ZHC DecodeConst make_decode_const(uint32_t h) {
+  DecodeConst c{};
+  const uint32_t w64 = root_of_unity(6), w64i = pow_mod_host(w64, P - 2);
+  uint32_t a = 1, b = 1;
+  for (int e = 0; e < 64; e++) {
+    c.roots[e] = tw_const(a);
+    c.roots[64 + e] = tw_const(b);
+    a = mul_mod_host(a, w64);
+    b = mul_mod_host(b, w64i);
+  }
+  decode_mid_plan<DecPlan8>(c, h);
+  decode_mid_plan<DecPlan9>(c, h);
+  decode_mid_plan<DecPlan10>(c, h);
+  decode_mid_plan<DecPlan11>(c, h);
+  decode_mid_plan<DecPlan12>(c, h);
+  return c;
+}
+
+// DecodeTabs over a copy `packed` of the image and the uniform tables `c`.
+inline DecodeTabs make_decode_tabs(int L, const DecodeTableImage& im, const uint32_t* packed, const DecodeConst& c) {
+  DecodeTabs T;
+  T.packed = packed;
+  T.o_inv1 = im.o_inv1;
+  T.o_fwd1 = im.o_fwd1;
+  T.o_fwd2 = im.o_fwd2;
+  T.o_inv2 = im.o_inv2;
+  T.packed_words = (uint32_t)im.packed.size();
+  for (int p = 0; p < 2; p++) {
+    T.o_fold[p] = im.o_fold[p];
+    T.mid[p] = c.mid[L - DEC_FP64_MIN_LOG][p];
+  }
+  T.roots = c.roots;
+  return T;
+}
+
+template <class F>
+int with_decode_plan(int L, F&& f) {
+  switch (L) {
+    case 8: return f(DecPlan8{});
+    case 9: return f(DecPlan9{});
+    case 10: return f(DecPlan10{});
+    case 11: return f(DecPlan11{});
+    case 12: return f(DecPlan12{});
+    default: return -1;
+  }
+}
+
+}  // namespace zoda
