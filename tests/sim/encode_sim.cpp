@@ -141,6 +141,12 @@ extern "C" int zoda_sim_encode(const uint32_t* in, uint32_t k, uint32_t cols, ui
   return zoda_sim_encode_variant(in, k, cols, n, out, layout, -1);
 }
 
+// The device code's reduction to canonical form, run on the host, for each of
+// `count` integer-valued doubles.
+extern "C" void zoda_sim_canonical(const double* x, size_t count, uint32_t* out) {
+  for (size_t i = 0; i < count; i++) out[i] = canonical(x[i]);
+}
+
 // Host runs of the device code's SHA-256 and RLC coefficient derivation.
 extern "C" void zoda_sim_sha256(const uint8_t* data, size_t len, uint8_t* out) {
   Sha256 h;

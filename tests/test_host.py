@@ -135,6 +135,27 @@ def test_persistent_kernel_emulation(k, cols):
     assert np.array_equal(sim_encode_persistent(data), O.encode(data, threads=4))
 
 
+def test_canonical_reduction_edges():
+    # the kernels' floor-division reduction: every multiple of p in range and
+    # its neighbours (j = 0, 1, p - 1 in x = n p + j), and random values over
+    # the whole input range |x| < 2^36, against Python's exact modulo
+    import ctypes
+    from helpers import sim_lib
+    lib = sim_lib()
+    lib.zoda_sim_canonical.restype = None
+    lib.zoda_sim_canonical.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    lim = 1 << 36
+    ns = np.arange(-(lim // P) - 1, lim // P + 2, dtype=np.int64)
+    xs = np.concatenate([ns * P, ns * P + 1, ns * P - 1, ns * P + (P - 1),
+                         np.random.default_rng(3).integers(-lim + 1, lim, size=200000)])
+    xs = xs[np.abs(xs) < lim]
+    d = xs.astype(np.float64)
+    assert np.array_equal(d.astype(np.int64), xs)
+    out = np.zeros(xs.size, dtype=np.uint32)
+    lib.zoda_sim_canonical(d.ctypes.data, xs.size, out.ctypes.data)
+    assert np.array_equal(out.astype(np.int64), np.mod(xs, P))
+
+
 @pytest.mark.parametrize("k,cols", [(128, 9), (256, 5), (512, 6), (1024, 3), (2048, 2)])
 def test_decoder_kernel_emulation(k, cols):
     # the FP64 column kernel of the erasure decoder, step by step on the host,

@@ -53,10 +53,19 @@ ZHD double reduce(double x) {
   return fmad(-q, PD, x);
 }
 
-// |x| < 2^51 -> canonical value in [0, p).
+// |x| < 2^36 (an integer) -> canonical value in [0, p), by floor division:
+//   y = x/p - 1/2 + 2^-40   one FMA; with |x/p| < 2^6 the error of 1/p adds
+//                           below 2^-47 and the rounding of y at most 2^-47
+//   q = round(y)            = floor(x/p): x/p = n + j/p with 0 <= j < p, so
+//                           y is within 2^-46 of a value between
+//                           n - 1/2 + 2^-40 and n + 1/2 - 1/p + 2^-40, and
+//                           1/p > 2^-31
+//   r = x - q p             in [0, p), exact.
+constexpr double FLOOR_BIAS = -0.5 + 0x1p-40;
 ZHD uint32_t canonical(double x) {
-  const int32_t r = (int32_t)reduce(x);  // |r| <= p/2 + 1, exact in int32
-  return (uint32_t)r + ((uint32_t)(r >> 31) & 2013265921u);
+  const double y = fmad(x, PINV, FLOOR_BIAS);
+  const double q = (y + ROUND_MAGIC) - ROUND_MAGIC;
+  return (uint32_t)fmad(-q, PD, x);
 }
 
 }  // namespace zoda
