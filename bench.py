@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--cpu-cols", type=int, default=16384, help="columns in the CPU sample")
     ap.add_argument("--no-batched", action="store_true", help="skip the 96-square batched comparison")
     ap.add_argument("--no-served", action="store_true", help="skip the served-trace comparison")
-    ap.add_argument("--served-requests", type=int, default=1000)
+    ap.add_argument("--served-requests", type=int, default=3000,
+                    help="requests per served trace (p99 rests on 1 %% of them)")
     ap.add_argument("--no-rlc", action="store_true", help="skip the RLC leg")
     ap.add_argument("--no-decode", action="store_true", help="skip the erasure-decode leg")
     ap.add_argument("--only-encode", action="store_true",
@@ -163,7 +164,7 @@ def batched_vs_plain(zoda_hip, torch, seed: int, reps: int = 7):
     }
 
 
-def served_vs_plain(zoda_hip, torch, seed: int, count: int, rates=(2000, 8000)):
+def served_vs_plain(zoda_hip, torch, seed: int, count: int, rates=(2000, 8000), slots: int = 3):
     """Poisson arrivals of squares with log-uniform sides {64..1024} (512 B
     shares, device-resident payloads drawn from a pool of 12 buffers per side),
     replayed in real time against the continuous-batching server (HBM budget
@@ -184,7 +185,7 @@ def served_vs_plain(zoda_hip, torch, seed: int, count: int, rates=(2000, 8000)):
     # its timed trace: a running server keeps those resident.
     warm = [(pool[sd][0][0].data_ptr(), pool[sd][0][1].data_ptr(), sd, 128 * sd, sd) for sd in sides_all]
     for mode in (0, 1):
-        zoda_hip.serve_trace(warm, np.zeros(len(warm)), hbm_budget=200 * 10**9, slots=3, mode=mode)
+        zoda_hip.serve_trace(warm, np.zeros(len(warm)), hbm_budget=200 * 10**9, slots=slots, mode=mode)
     out = {}
     for rate in rates:
         rng = np.random.default_rng(seed + rate)
@@ -198,7 +199,7 @@ def served_vs_plain(zoda_hip, torch, seed: int, count: int, rates=(2000, 8000)):
         payload = sum(sd * sd * SHARE_BYTES for sd in sides)
         res = {}
         for mode, name in ((0, "served"), (1, "plain_loop")):
-            lat, st = zoda_hip.serve_trace(specs, arrivals, hbm_budget=200 * 10**9, slots=3, mode=mode)
+            lat, st = zoda_hip.serve_trace(specs, arrivals, hbm_budget=200 * 10**9, slots=slots, mode=mode)
             res[name] = {
                 "p50_us": round(float(np.percentile(lat, 50)), 1),
                 "p99_us": round(float(np.percentile(lat, 99)), 1),
@@ -210,7 +211,7 @@ def served_vs_plain(zoda_hip, torch, seed: int, count: int, rates=(2000, 8000)):
         out[f"{rate}_req_per_s"] = res
     return {
         "workload": f"{count} requests, Poisson arrivals, sides log-uniform {{64..1024}}, 512 B shares, "
-                    "HBM budget 200 GB, 3 slots; Gb/s = payload over first arrival to last completion",
+                    f"HBM budget 200 GB, {slots} slots; Gb/s = payload over first arrival to last completion",
         **out,
     }
 

@@ -5,11 +5,14 @@
 // counter.  Whenever a slot is free it takes the next launch:
 //   * a large request (side >= 2^DIRECT_MIN_LOG: its own fused kernel fills
 //     the device) at the head of the queue goes alone through the single-square
-//     kernel; at most slots - 1 of them run at once, in arrival order;
-//   * otherwise every queued small request that fits the HBM budget next to
-//     the requests on the GPU goes out as one persistent batched launch whose
-//     descriptors travel in the kernel arguments (nothing is copied before
-//     the launch; the counter is never reset).
+//     kernel; at most slots - 1 of them run at once, in arrival order; while
+//     the head is a large request that has to wait (for a slot or memory),
+//     nothing passes it;
+//   * otherwise (a small request at the head) every queued small request
+//     that fits the HBM budget next to the requests on the GPU goes out as
+//     one persistent batched launch whose descriptors travel in the kernel
+//     arguments (nothing is copied before the launch; the counter is never
+//     reset).
 // Requests that arrive while others run are admitted as soon as a slot and
 // memory free up, and their launches overlap the ones in flight.  Twiddle
 // tables stay resident in the per-device cache.  Completion times come from
@@ -216,12 +219,12 @@ class Server {
   }
 
  private:
-  // The next launch for free slot s (see the top of this file).  Large
-  // requests keep arrival order; small ones may pass a large head that waits
-  // for a slot or memory, but not into the memory the head needs.
+  // The next launch for free slot s (see the top of this file).  A large head
+  // that waits blocks the queue: letting small requests pass it put their
+  // batches on the GPU ahead of it and raised the p99 latency (the large
+  // requests are the tail).
   int fill_slot(Slot& s, bool* launched) {
     *launched = false;
-    uint64_t reserve = 0;
     const uint64_t head = queue_.front();
     if (reqs_[head].big) {
       Request& r = reqs_[head];
@@ -234,7 +237,7 @@ class Server {
         big_in_flight_++;
         return commit(s, std::vector<uint64_t>{head}, r.bytes, launched);
       }
-      reserve = r.bytes;
+      return 0;
     }
     ServeBatch b{};
     std::vector<uint64_t> taken;
@@ -245,7 +248,7 @@ class Server {
         ++it;
         continue;
       }
-      if (in_flight_bytes_ + reserve + bytes + r.bytes > budget_) break;
+      if (in_flight_bytes_ + bytes + r.bytes > budget_) break;
       bytes += r.bytes;
       taken.push_back(*it);
       it = queue_.erase(it);
