@@ -228,7 +228,7 @@ int zoda_hip_ntt_mod(uint32_t* values, const uint32_t* roots, uint32_t n, uint32
  * d_enc: 2k x cols, rows not listed are never read.  known_rows: host array of
  * row indices (the first k distinct ones are used).  d_data: k x cols.  k a
  * power of two <= 16384 (one column of 2k values is transformed in LDS).
- * Scratch: 4k * cols bytes of device memory for 128 <= k <= 2048, 8k * cols
+ * Scratch: 4k * cols bytes of device memory for 128 <= k <= 8192, 8k * cols
  * otherwise.  Synchronous on `stream`. */
 int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_t cols, const uint32_t* known_rows,
                            uint32_t nknown, uint32_t* d_data, void* stream);

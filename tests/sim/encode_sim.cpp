@@ -216,7 +216,8 @@ extern "C" int zoda_sim_decode(const uint32_t* enc, uint32_t k, uint32_t cols, c
       T[(size_t)c * k + i] = mul_mod_host(enc[(size_t)rows[i] * cols + c], zk[rows[i]]);
   const int rc = with_decode_plan((int)L, [&](auto plan) -> int {
     using PL = decltype(plan);
-    const DecodeTableImage im = build_decode_tables(PL::L, PL::R1, PL::R2, PL::RD, h);
+    const DecodeTableImage im = build_decode_tables<PL>(h);
+    if (im.packed.empty()) return -3;
     static const DecodeConst uniform = make_decode_const(31);  // the kernel's constant-memory tables
     const DecodeTabs tabs = make_decode_tabs(PL::L, im, im.packed.data(), uniform);
     std::vector<int32_t> lds(PL::LDS_WORDS);

@@ -29,7 +29,7 @@ def patterns(k, rng):
     yield "random", rng.choice(n, size=k, replace=False)
 
 
-@pytest.mark.parametrize("k,cols", [(1, 5), (2, 33), (64, 1000), (256, 4097), (1024, 300), (4096, 64), (16384, 3)])
+@pytest.mark.parametrize("k,cols", [(1, 5), (2, 33), (64, 1000), (128, 77), (256, 4097), (512, 130), (1024, 300), (4096, 64), (8192, 5), (16384, 3)])
 def test_decode_from_any_k_rows(zh, torch, k, cols):
     rng = np.random.default_rng(k * 7 + cols)
     t_in = torch.empty((k, cols), dtype=torch.int32, device="cuda")

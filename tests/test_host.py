@@ -156,7 +156,7 @@ def test_canonical_reduction_edges():
     assert np.array_equal(out.astype(np.int64), np.mod(xs, P))
 
 
-@pytest.mark.parametrize("k,cols", [(128, 9), (256, 5), (512, 6), (1024, 3), (2048, 2)])
+@pytest.mark.parametrize("k,cols", [(128, 9), (256, 5), (512, 6), (1024, 3), (2048, 2), (4096, 1), (8192, 1)])
 def test_decoder_kernel_emulation(k, cols):
     # the FP64 column kernel of the erasure decoder, step by step on the host,
     # from the odd rows, a random half, and the last k rows; ragged last tiles

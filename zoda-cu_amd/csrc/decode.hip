@@ -194,11 +194,11 @@ __global__ __launch_bounds__(DEC_NT) void dec_fused(uint32_t* __restrict__ T, ui
 
 __constant__ DecodeConst kDecodeConst = make_decode_const(DEC_COSET);
 
-// Persistent form for 2^8 <= n <= 2^12: tiles of W columns of the
+// Persistent form for 2^8 <= n <= 2^14: tiles of W columns of the
 // rank-packed scratch (k values each) through both FP64 transform pairs in
 // LDS (decode_core.hpp); the k data values of each column replace its entries.
 template <class PL>
-__global__ __launch_bounds__(PL::NT, 4) void dec_columns(uint32_t* __restrict__ T, uint32_t cols, uint32_t ntiles,
+__global__ __launch_bounds__(PL::NT, PL::MINW) void dec_columns(uint32_t* __restrict__ T, uint32_t cols, uint32_t ntiles,
                                                       const uint32_t* __restrict__ rows,
                                                       const uint32_t* __restrict__ zinv, DecodeTabs tabs) {
   extern __shared__ int32_t lds[];
@@ -242,12 +242,15 @@ __global__ __launch_bounds__(PL::NT, 4) void dec_columns(uint32_t* __restrict__ 
   }
 }
 
-// Tables of the FP64 decoder for n = 2^L, built and uploaded once per device.
-int get_decode_tables(DeviceState* st, int L, int r1, int r2, int rd, const DecodeTabs** out) {
+// Tables of the FP64 decoder plan PL, built and uploaded once per device.
+template <class PL>
+int get_decode_tables(DeviceState* st, const DecodeTabs** out) {
+  constexpr int L = PL::L;
   std::lock_guard<std::recursive_mutex> lock(st->mu);
   auto it = st->decode.find((uint32_t)L);
   if (it == st->decode.end()) {
-    const DecodeTableImage im = build_decode_tables(L, r1, r2, rd, DEC_COSET);
+    const DecodeTableImage im = build_decode_tables<PL>(DEC_COSET);
+    if (im.packed.empty()) return fail(ZODA_ESTATE, "decoder table layout mismatch for n = 2^%d", L);
     DecodeTableSet set;
     if (hipMalloc((void**)&set.packed, im.packed.size() * sizeof(uint32_t)) != hipSuccess) {
       (void)hipGetLastError();
@@ -269,13 +272,16 @@ int get_decode_tables(DeviceState* st, int L, int r1, int r2, int rd, const Deco
   return 0;
 }
 
-// Once per device: the persistent grid of dec_columns<PL> (every CU filled).
+// Once per device: the LDS limit of dec_columns<PL> and its persistent grid
+// (every CU filled).
 template <class PL>
 int decode_grid(DeviceState* st, uint32_t* grid) {
   const void* fn = (const void*)dec_columns<PL>;
   std::lock_guard<std::recursive_mutex> lock(st->mu);
   auto it = st->kernel_setup.find(fn);
   if (it == st->kernel_setup.end()) {
+    constexpr size_t lds = PL::LDS_WORDS * sizeof(int32_t);
+    if (lds > 64 * 1024) ZODA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0, cus = 0;
     ZODA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, PL::NT, PL::LDS_WORDS * sizeof(int32_t)));
     ZODA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, st->device));
@@ -327,7 +333,7 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
   if (have < k) return fail(ZODA_EINVAL, "%u distinct known rows, %u needed", have, k);
   DeviceState* st = nullptr;
   ZODA_TRY(current_state(&st));
-  // 2^8 <= n <= 2^12: the FP64 column kernel; other sizes: the Montgomery one.
+  // 2^8 <= n <= 2^14: the FP64 column kernel; other sizes: the Montgomery one.
   const bool fp64 = (int)log_n >= DEC_FP64_MIN_LOG && (int)log_n <= DEC_FP64_MAX_LOG;
   MontTables tn, tk;
   const DecodeTabs* dtabs = nullptr;
@@ -335,7 +341,7 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
   if (fp64) {
     ZODA_TRY(with_decode_plan((int)log_n, [&](auto plan) -> int {
       using PL = decltype(plan);
-      ZODA_TRY(get_decode_tables(st, PL::L, PL::R1, PL::R2, PL::RD, &dtabs));
+      ZODA_TRY(get_decode_tables<PL>(st, &dtabs));
       return decode_grid<PL>(st, &dgrid);
     }));
   } else {

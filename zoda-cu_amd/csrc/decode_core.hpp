@@ -44,9 +44,10 @@ struct DecodeTabs {
   const Tw* roots;   // w_64^e for e < 64, then w_64^-e
 };
 
-template <int L_, int W_, int NT_, int R1_, int R2_, int RD_>
+template <int L_, int W_, int NT_, int R1_, int R2_, int RD_, int MINW_ = 4>
 struct DecodePlan {
   static constexpr int L = L_, W = W_, NT = NT_, R1 = R1_, R2 = R2_, RD = RD_;
+  static constexpr int MINW = MINW_;  // waves per SIMD the kernel is compiled for
   static_assert(R1 > 0 && R2 > 0 && RD > 0 && R1 <= 5 && R2 <= 5 && RD <= 5, "three register digits");
   static_assert(R1 + R2 + RD == L, "digits must cover the row index");
   static constexpr int S = 1 << L;  // n
@@ -55,14 +56,34 @@ struct DecodePlan {
   // pass's lanes (2^RD rows apart) fall into different banks.
   static constexpr int TILE_VALUES = W * S;
   static constexpr int TILE_WORDS = W * S + (S >> RD);
-  // inv1, fwd1 (n each), fwd2, inv2 (2^(LO2+R2) each), fold[2] (2^(R1+R2) each)
-  static constexpr int TW_WORDS = 2 * S + 2 * (1 << (LO2 + R2)) + 2 * (1 << (R1 + R2));
+  // The top digit's twiddles w^(+-low k), low < 2^LO1: one table of n
+  // entries, or (FT, n >= 2^13) two factors of about sqrt(n) entries each,
+  // low = lh 2^FA + ll: [k << FA | ll] then [k << (LO1 - FA) | lh].
+  static constexpr bool FT = L >= 13;
+  static constexpr int FA = FT ? (LO1 + 1) / 2 : LO1;
+  static constexpr int TOP_WORDS = FT ? (1 << (R1 + FA)) + (1 << (R1 + LO1 - FA)) : S;
+  // inv1, fwd1 (TOP_WORDS each), fwd2, inv2 (2^(LO2+R2) each), fold[2] (2^(R1+R2) each)
+  static constexpr int TW_WORDS = 2 * TOP_WORDS + 2 * (1 << (LO2 + R2)) + 2 * (1 << (R1 + R2));
   static constexpr int LDS_WORDS = TILE_WORDS + TW_WORDS;
 };
 
 template <class PL>
 ZHD uint32_t lds_at(uint32_t row, uint32_t c) {
   return row * PL::W + c + (row >> PL::RD);
+}
+
+// x * w^(+-low k) for the top digit's table(s) `t` (see DecodePlan::FT).
+template <class PL>
+ZHD double top_twiddle(double x, const uint32_t* __restrict__ t, uint32_t k, uint32_t low) {
+  const TwTab tab{nullptr, t};
+  if constexpr (PL::FT) {
+    constexpr int FA = PL::FA, HB = PL::LO1 - PL::FA;
+    const Tw a = tw_get<true>(tab, (k << FA) | (low & ((1u << FA) - 1u)));
+    const Tw b = tw_get<true>(tab, (1u << (PL::R1 + FA)) + ((k << HB) | (low >> FA)));
+    return mulmod(mulmod(x, a), b);
+  } else {
+    return mulmod(x, tw_get<true>(tab, (k << PL::LO1) | low));
+  }
 }
 
 // INTT pass on the digit of R bits at bit LO: inverse DFT over the digit,
@@ -95,7 +116,7 @@ ZHD void dec_intt(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
       static_assert(FOLDED || LO + R == PL::L, "the unfolded pass is the top digit's");
       x[0] = reduce(x[0]);  // twiddle w^0 = 1
 #pragma unroll
-      for (int k = 1; k < RR; k++) x[k] = mulmod(x[k], tw_get<true>(own, ((uint32_t)k << LO) | low));
+      for (int k = 1; k < RR; k++) x[k] = top_twiddle<PL>(x[k], tl, (uint32_t)k, low);
     }
 #pragma unroll
     for (int m = 0; m < RR; m++) lds[lds_at<PL>(row_of<LO, R>(u, m), c)] = (int32_t)x[m];
@@ -140,7 +161,6 @@ ZHD void dec_ntt2(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
   constexpr int R = PL::R2, RR = 1 << R, LO = PL::LO2, LOA = PL::LO1, RA = PL::R1;
   constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
   static_assert((PL::W << (PL::L - R)) % PL::NT == 0, "groups must divide evenly over the threads");
-  const TwTab above{nullptr, ta};
 #pragma unroll
   for (int gi = 0; gi < G; gi++) {
     group_fence();
@@ -153,8 +173,7 @@ ZHD void dec_ntt2(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
 #pragma unroll
     for (int m = 0; m < RR; m++) {
       const uint32_t j = row_of<LO, R>(u, m);
-      const uint32_t k = (j >> LOA) & ((1u << RA) - 1u);
-      x[m] = mulmod(x[m], tw_get<true>(above, (k << LOA) | (j & ((1u << LOA) - 1u))));
+      x[m] = top_twiddle<PL>(x[m], ta, (j >> LOA) & ((1u << RA) - 1u), j & ((1u << LOA) - 1u));
     }
 #pragma unroll
     for (int m = 0; m < RR; m++) lds[lds_at<PL>(row_of<LO, R>(u, m), c)] = (int32_t)x[m];
@@ -279,14 +298,34 @@ inline void push_inverse_digit(std::vector<Tw>& v, int lo, int r) {
   }
 }
 
-inline DecodeTableImage build_decode_tables(int L, int r1, int r2, int rd, uint32_t h) {
+// The top digit's twiddles w^(+-low k) as DecodePlan lays them out: one
+// [k << lo | low] table, or (factored) the two tables of the low FA bits and
+// the high bits of `low`.
+inline void push_top_digit(std::vector<Tw>& v, int lo, int r, bool inverse, bool factored, int fa) {
+  uint32_t w = root_of_unity((uint32_t)(lo + r));
+  if (inverse) w = inv_mod_host(w);
+  if (!factored) {
+    for (uint32_t k = 0; k < (1u << r); k++)
+      for (uint32_t low = 0; low < (1u << lo); low++) v.push_back(make_tw(pow_mod_host(w, (uint64_t)low * k)));
+    return;
+  }
+  for (uint32_t k = 0; k < (1u << r); k++)
+    for (uint32_t ll = 0; ll < (1u << fa); ll++) v.push_back(make_tw(pow_mod_host(w, (uint64_t)ll * k)));
+  for (uint32_t k = 0; k < (1u << r); k++)
+    for (uint32_t lh = 0; lh < (1u << (lo - fa)); lh++)
+      v.push_back(make_tw(pow_mod_host(w, ((uint64_t)lh << fa) * k)));
+}
+
+template <class PL>
+DecodeTableImage build_decode_tables(uint32_t h) {
   DecodeTableImage im;
+  constexpr int L = PL::L, r1 = PL::R1, r2 = PL::R2, rd = PL::RD;
   const int lo1 = rd + r2, lo2 = rd;
   std::vector<Tw> t;
-  push_inverse_digit(t, lo1, r1);
+  push_top_digit(t, lo1, r1, true, PL::FT, PL::FA);
   im.o_inv1 = 0;
   im.o_fwd1 = (uint32_t)t.size();
-  push_digit_table(t, lo1, r1, false);
+  push_top_digit(t, lo1, r1, false, PL::FT, PL::FA);
   im.o_fwd2 = (uint32_t)t.size();
   push_digit_table(t, lo2, r2, false);
   im.o_inv2 = (uint32_t)t.size();
@@ -298,6 +337,7 @@ inline DecodeTableImage build_decode_tables(int L, int r1, int r2, int rd, uint3
     push_fold_table(t, r1, r2, g[p], n_inv);
   }
   for (const Tw& x : t) im.packed.push_back((uint32_t)x.w);
+  if (im.packed.size() != (size_t)PL::TW_WORDS) im.packed.clear();  // a layout mismatch: no tables
   return im;
 }
 
@@ -305,7 +345,7 @@ inline DecodeTableImage build_decode_tables(int L, int r1, int r2, int rd, uint3
 
 struct DecodeConst {
   Tw roots[128];     // w_64^e for e < 64, then w_64^-e
-  Tw mid[5][2][32];  // per plan (L = 8..12) and pair: the middle pass's twist factors
+  Tw mid[7][2][32];  // per plan (L = 8..14) and pair: the middle pass's twist factors
 };
 
 ZHC Tw tw_const(uint32_t w) { return Tw{(double)w, (double)w / PD}; }
@@ -323,15 +363,19 @@ ZHC void decode_mid(Tw* out, int r1, int r2, int rd, uint32_t h, int p) {
   }
 }
 
-// Decoder plans: n = 2^L for 8 <= L <= 12, 512 threads, W columns so that
-// every pass has at least one group per thread (two work-groups per CU).
-//                      L   W   NT  R1 R2 RD
+// Decoder plans: n = 2^L for 8 <= L <= 14, W columns so that every pass has
+// at least one group per thread; up to n = 2^12 two 512-thread work-groups
+// per CU, then the work-groups LDS allows (three at 2^13, one at 2^14), each
+// compiled for at most that many waves per SIMD.
+//                      L   W   NT  R1 R2 RD  MINW
 using DecPlan8 = DecodePlan<8, 16, 512, 3, 3, 2>;
 using DecPlan9 = DecodePlan<9, 8, 512, 3, 3, 3>;
 using DecPlan10 = DecodePlan<10, 8, 512, 4, 3, 3>;
 using DecPlan11 = DecodePlan<11, 4, 512, 4, 4, 3>;
 using DecPlan12 = DecodePlan<12, 2, 512, 4, 4, 4>;
-constexpr int DEC_FP64_MIN_LOG = 8, DEC_FP64_MAX_LOG = 12;
+using DecPlan13 = DecodePlan<13, 1, 256, 5, 4, 4, 2>;
+using DecPlan14 = DecodePlan<14, 1, 512, 5, 5, 4, 2>;
+constexpr int DEC_FP64_MIN_LOG = 8, DEC_FP64_MAX_LOG = 14;
 
 template <class PL>
 ZHC void decode_mid_plan(DecodeConst& c, uint32_t h) {
@@ -353,6 +397,8 @@ ZHC DecodeConst make_decode_const(uint32_t h) {
   decode_mid_plan<DecPlan10>(c, h);
   decode_mid_plan<DecPlan11>(c, h);
   decode_mid_plan<DecPlan12>(c, h);
+  decode_mid_plan<DecPlan13>(c, h);
+  decode_mid_plan<DecPlan14>(c, h);
   return c;
 }
 
@@ -381,6 +427,8 @@ int with_decode_plan(int L, F&& f) {
     case 10: return f(DecPlan10{});
     case 11: return f(DecPlan11{});
     case 12: return f(DecPlan12{});
+    case 13: return f(DecPlan13{});
+    case 14: return f(DecPlan14{});
     default: return -1;
   }
 }
