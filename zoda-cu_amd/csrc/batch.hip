@@ -18,11 +18,15 @@
 
 namespace zoda {
 
-// Not inlined: each plan is register-allocated on its own, so the kernel's
-// VGPR budget is the largest plan's rather than a merge of all of them.
+// Two instantiations of each batched kernel: one for every side, and one for
+// the sides the queue holds when the large squares run as their own launches
+// (below 2^DIRECT_MIN_LOG).  A kernel's register budget is its largest
+// plan's, so the small one fits several times more work-groups on a CU.
+constexpr int BATCH_SMALL_LOG = DIRECT_MIN_LOG - 1;
+
 template <class PL, bool FULL>
-__device__ __attribute__((noinline)) void batch_tile_steps(int tid, uint32_t tile, const SquareDesc& d,
-                                                 int32_t* lds, const EncodeTabs& T, bool load) {
+__device__ __forceinline__ void batch_tile_body(int tid, uint32_t tile, const SquareDesc& d, int32_t* lds,
+                                                const EncodeTabs& T, bool load) {
   if constexpr (PL::TWL) {
     if (load) {  // uniform: the work-group changed plan since its last tile
       encode_load_tables<PL>(tid, lds, T);
@@ -44,12 +48,24 @@ __device__ __attribute__((noinline)) void batch_tile_steps(int tid, uint32_t til
   }
 }
 
+// Not inlined: each plan is register-allocated on its own, so a kernel's VGPR
+// budget is its largest plan's rather than a merge of all of them.  MAXL
+// keeps the instances each kernel calls apart, so the small-side kernel's
+// occupancy bound reaches its plans.
+template <class PL, bool FULL, int MAXL>
+__device__ __attribute__((noinline)) void batch_tile_steps(int tid, uint32_t tile, const SquareDesc& d,
+                                                           int32_t* lds, const EncodeTabs& T, bool load) {
+  batch_tile_body<PL, FULL>(tid, tile, d, lds, T, load);
+}
+
 __global__ void batch_reset(uint32_t* counter) { *counter = 0u; }
 
 // All passes of column tile `tile` of square d with the batched plan of its
-// side.  *loaded names the plan whose twiddle tables are in LDS (-1: none);
-// they are reloaded when the plan changes.  The caller's barrier after the
-// previous tile separates its last reads of LDS from the reload.
+// side (sides up to 2^MAXL; the launcher picks the kernel).  *loaded names the
+// plan whose twiddle tables are in LDS (-1: none); they are reloaded when the
+// plan changes.  The caller's barrier after the previous tile separates its
+// last reads of LDS from the reload.
+template <int MAXL>
 __device__ __forceinline__ void run_batch_tile(int tid, const SquareDesc& d, uint32_t tile,
                                                int32_t* lds, const EncodeTabs& T, int* loaded) {
   const int key = (int)(d.L * BATCH_VARIANTS + d.variant);
@@ -57,10 +73,12 @@ __device__ __forceinline__ void run_batch_tile(int tid, const SquareDesc& d, uin
   *loaded = key;
   with_batch_plan((int)d.L, (int)d.variant, [&](auto plan) -> int {
     using PL = decltype(plan);
-    if (d.full)
-      batch_tile_steps<PL, true>(tid, tile, d, lds, T, load);
-    else
-      batch_tile_steps<PL, false>(tid, tile, d, lds, T, load);
+    if constexpr (PL::L <= MAXL) {
+      if (d.full)
+        batch_tile_steps<PL, true, MAXL>(tid, tile, d, lds, T, load);
+      else
+        batch_tile_steps<PL, false, MAXL>(tid, tile, d, lds, T, load);
+    }
     return 0;
   });
 }
@@ -76,7 +94,12 @@ __device__ __forceinline__ uint32_t take_item(int tid, uint32_t* counter, uint32
   return it;
 }
 
-__global__ __launch_bounds__(BATCH_NT) void batch_kernel(const SquareDesc* __restrict__ squares,
+// Waves per SIMD each instantiation is compiled for.
+template <int MAXL>
+constexpr int batch_min_waves() { return MAXL <= BATCH_SMALL_LOG ? 4 : 1; }
+
+template <int MAXL>
+__global__ __launch_bounds__(BATCH_NT, batch_min_waves<MAXL>()) void batch_kernel(const SquareDesc* __restrict__ squares,
                                                          const BatchItem* __restrict__ items,
                                                          uint32_t nitems, uint32_t* counter,
                                                          const EncodeTabs* __restrict__ tabs) {
@@ -88,11 +111,12 @@ __global__ __launch_bounds__(BATCH_NT) void batch_kernel(const SquareDesc* __res
     if (it >= nitems) break;
     const BatchItem bi = items[it];
     const SquareDesc d = squares[bi.square];
-    run_batch_tile(tid, d, bi.tile, lds, tabs[d.L * BATCH_VARIANTS + d.variant], &loaded);
+    run_batch_tile<MAXL>(tid, d, bi.tile, lds, tabs[d.L * BATCH_VARIANTS + d.variant], &loaded);
   }
 }
 
-__global__ __launch_bounds__(BATCH_NT) void serve_batch_kernel(const ServeBatch b) {
+template <int MAXL>
+__global__ __launch_bounds__(BATCH_NT, batch_min_waves<MAXL>()) void serve_batch_kernel(const ServeBatch b) {
   extern __shared__ int32_t lds[];
   const int tid = (int)threadIdx.x;
   int loaded = -1;
@@ -102,38 +126,60 @@ __global__ __launch_bounds__(BATCH_NT) void serve_batch_kernel(const ServeBatch 
     uint32_t q = 0;
     while (q + 1 < b.n && b.first[q + 1] <= it) q++;
     const SquareDesc d = b.sq[q];
-    run_batch_tile(tid, d, it - b.first[q], lds, b.tabs[d.L * BATCH_VARIANTS + d.variant], &loaded);
+    run_batch_tile<MAXL>(tid, d, it - b.first[q], lds, b.tabs[d.L * BATCH_VARIANTS + d.variant], &loaded);
   }
 }
 
-int batch_grid(DeviceState* st, uint32_t* grid) {
-  int per_cu = 0, cus = 0;
-  // Both batched kernels take more than the default 64 KiB of dynamic LDS.
-  ZODA_HIP(hipFuncSetAttribute((const void*)batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               BATCH_LDS_WORDS * 4));
-  ZODA_HIP(hipFuncSetAttribute((const void*)serve_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               BATCH_LDS_WORDS * 4));
-  ZODA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel, BATCH_NT,
-                                                         BATCH_LDS_WORDS * 4));
-  ZODA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, st->device));
-  *grid = (uint32_t)std::max(1, per_cu * cus);
+uint32_t batch_lds_bytes(uint32_t L, uint32_t variant) {
+  const int words = with_batch_plan((int)L, (int)variant, [](auto plan) -> int {
+    return decltype(plan)::LDS_WORDS;
+  });
+  return (uint32_t)std::max(words, 1) * 4;
+}
+
+int batch_grid(DeviceState* st, uint32_t lds_bytes, uint32_t max_log, uint32_t* grid) {
+  const bool small = (int)max_log <= BATCH_SMALL_LOG;
+  const uint32_t key = lds_bytes | (small ? 0x80000000u : 0u);
+  std::lock_guard<std::recursive_mutex> lock(st->mu);
+  auto it = st->batch_grid.find(key);
+  if (it == st->batch_grid.end()) {
+    int per_cu = 0, cus = 0;
+    // The largest plans take more than the default 64 KiB of dynamic LDS.
+    const void* fns[4] = {(const void*)batch_kernel<BATCH_MAX_LOG>, (const void*)serve_batch_kernel<BATCH_MAX_LOG>,
+                          (const void*)batch_kernel<BATCH_SMALL_LOG>,
+                          (const void*)serve_batch_kernel<BATCH_SMALL_LOG>};
+    for (const void* fn : fns)
+      ZODA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, BATCH_LDS_WORDS * 4));
+    ZODA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, small ? fns[2] : fns[0], BATCH_NT, lds_bytes));
+    ZODA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, st->device));
+    it = st->batch_grid.emplace(key, (uint32_t)std::max(1, per_cu * cus)).first;
+  }
+  *grid = it->second;
   return 0;
 }
 
 int launch_batch_items(const SquareDesc* d_squares, const BatchItem* d_items, uint32_t nitems,
-                       uint32_t* d_counter, const EncodeTabs* d_tabs, uint32_t grid,
-                       hipStream_t stream) {
+                       uint32_t* d_counter, const EncodeTabs* d_tabs, uint32_t grid, uint32_t lds_bytes,
+                       uint32_t max_log, hipStream_t stream) {
   hipLaunchKernelGGL(batch_reset, dim3(1), dim3(1), 0, stream, d_counter);
   ZODA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(batch_kernel, dim3(std::min(grid, nitems)), dim3(BATCH_NT), BATCH_LDS_WORDS * 4,
-                     stream, d_squares, d_items, nitems, d_counter, d_tabs);
+  if ((int)max_log <= BATCH_SMALL_LOG)
+    hipLaunchKernelGGL(batch_kernel<BATCH_SMALL_LOG>, dim3(std::min(grid, nitems)), dim3(BATCH_NT), lds_bytes,
+                       stream, d_squares, d_items, nitems, d_counter, d_tabs);
+  else
+    hipLaunchKernelGGL(batch_kernel<BATCH_MAX_LOG>, dim3(std::min(grid, nitems)), dim3(BATCH_NT), lds_bytes,
+                       stream, d_squares, d_items, nitems, d_counter, d_tabs);
   ZODA_HIP(hipGetLastError());
   return 0;
 }
 
-int launch_serve_batch(const ServeBatch& b, uint32_t grid, hipStream_t stream, uint32_t* advance) {
+int launch_serve_batch(const ServeBatch& b, uint32_t grid, uint32_t lds_bytes, uint32_t max_log,
+                       hipStream_t stream, uint32_t* advance) {
   const uint32_t g = std::min(grid, b.items);
-  hipLaunchKernelGGL(serve_batch_kernel, dim3(g), dim3(BATCH_NT), BATCH_LDS_WORDS * 4, stream, b);
+  if ((int)max_log <= BATCH_SMALL_LOG)
+    hipLaunchKernelGGL(serve_batch_kernel<BATCH_SMALL_LOG>, dim3(g), dim3(BATCH_NT), lds_bytes, stream, b);
+  else
+    hipLaunchKernelGGL(serve_batch_kernel<BATCH_MAX_LOG>, dim3(g), dim3(BATCH_NT), lds_bytes, stream, b);
   ZODA_HIP(hipGetLastError());
   *advance = b.items + g;
   return 0;
@@ -183,7 +229,7 @@ struct zoda_batch_plan {
   BatchItem* d_items = nullptr;
   EncodeTabs* d_tabs = nullptr;
   uint32_t* d_counter = nullptr;
-  uint32_t nitems = 0, grid = 0;
+  uint32_t nitems = 0, grid = 0, lds_bytes = 4, max_log = 0;
   // Squares that run as their own fused launches, largest first, on a branch
   // (side stream, fork/join events) parallel to the work-queue kernel.
   std::vector<zoda_square> direct;
@@ -222,7 +268,7 @@ static int launch_batch_direct(zoda_batch_plan* p, hipStream_t stream) {
     ZODA_TRY(launch_encode(st, q.in, q.k, q.cols, q.n, q.out, ZODA_LAYOUT_ROW, ds));
   if (queue)
     ZODA_TRY(launch_batch_items(p->d_squares, p->d_items, p->nitems, p->d_counter, p->d_tabs, p->grid,
-                                stream));
+                                p->lds_bytes, p->max_log, stream));
   if (queue && !p->direct.empty()) {
     ZODA_HIP(hipEventRecord(p->join, ds));
     ZODA_HIP(hipStreamWaitEvent(stream, p->join, 0));
@@ -303,7 +349,11 @@ static int build_plan(const zoda_square* squares, uint32_t count, bool graph, bo
   if (e == hipSuccess) e = hipMemcpy(p->d_tabs, tabs.data(), tabs.size() * sizeof(EncodeTabs), hipMemcpyHostToDevice);
   if (e != hipSuccess) return bail(fail((int)e, "batch plan upload failed: %s", hipGetErrorString(e)));
 
-  int rc = batch_grid(st, &p->grid);
+  for (const SquareDesc& d : descs) {
+    p->lds_bytes = std::max(p->lds_bytes, batch_lds_bytes(d.L, d.variant));
+    p->max_log = std::max(p->max_log, d.L);
+  }
+  int rc = batch_grid(st, p->lds_bytes, p->max_log, &p->grid);
   if (rc) return bail(rc);
   p->grid = std::max<uint32_t>(1, std::min<uint32_t>(p->grid, p->nitems));
 

@@ -39,15 +39,21 @@ int check_batch_square(const zoda_square& s, uint32_t i);
 // (side, variant) table pointers into tabs_by_slot[L * 2 + variant] once.
 int describe_square(DeviceState* st, const zoda_square& s, SquareDesc* d, EncodeTabs* tabs_by_slot,
                     bool* have_slot);
-// Persistent grid that fills every CU.
-int batch_grid(DeviceState* st, uint32_t* grid);
+// Dynamic LDS bytes of the batched plan of side 2^L, variant.  A launch takes
+// the largest over its squares: small sides need far less than the largest
+// plan, and more of their work-groups then fit a CU.
+uint32_t batch_lds_bytes(uint32_t L, uint32_t variant);
+// Persistent grid that fills every CU with work-groups of `lds_bytes`, for
+// the kernel of squares of side up to 2^max_log.
+int batch_grid(DeviceState* st, uint32_t lds_bytes, uint32_t max_log, uint32_t* grid);
 // [counter reset, persistent kernel] on `stream`.
 int launch_batch_items(const SquareDesc* d_squares, const BatchItem* d_items, uint32_t nitems,
-                       uint32_t* d_counter, const EncodeTabs* d_tabs, uint32_t grid,
-                       hipStream_t stream);
+                       uint32_t* d_counter, const EncodeTabs* d_tabs, uint32_t grid, uint32_t lds_bytes,
+                       uint32_t max_log, hipStream_t stream);
 
 // Persistent kernel over a ServeBatch on `stream`, on min(grid, items)
 // work-groups; *advance receives how far it moves the counter.
-int launch_serve_batch(const ServeBatch& b, uint32_t grid, hipStream_t stream, uint32_t* advance);
+int launch_serve_batch(const ServeBatch& b, uint32_t grid, uint32_t lds_bytes, uint32_t max_log,
+                       hipStream_t stream, uint32_t* advance);
 
 }  // namespace zoda

@@ -74,6 +74,7 @@ struct DeviceState {
   std::map<uint32_t, EncodeTableSet> encode;     // keyed by encode_table_key()
   std::map<uint32_t, DecodeTableSet> decode;     // keyed by log n
   int batch_variant[16] = {0};                   // autotuned batched-plan variant per log side
+  std::map<uint32_t, uint32_t> batch_grid;       // batched kernels' persistent grid by LDS bytes
   // Drop-in entry points: tables built from the caller's roots.
   std::map<uint32_t, std::vector<uint32_t>> user_roots;
   std::map<uint32_t, MontTables> user_mont;

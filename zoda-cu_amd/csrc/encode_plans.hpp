@@ -72,7 +72,6 @@ using B2a = EncodePlan<2, 256, BATCH_NT, 0, 0, 2, true>;
 using B3a = EncodePlan<3, 256, BATCH_NT, 0, 0, 3, true>;
 using B4a = EncodePlan<4, 256, BATCH_NT, 0, 0, 4, true>;
 using B4b = EncodePlan<4, 64, BATCH_NT, 2, 0, 2, true>;
-using B5a = EncodePlan<5, 256, BATCH_NT, 0, 0, 5, true>;
 using B5b = EncodePlan<5, 64, BATCH_NT, 3, 0, 2, true>;
 using B6a = EncodePlan<6, 64, BATCH_NT, 3, 0, 3, true>;
 using B6b = EncodePlan<6, 32, BATCH_NT, 3, 0, 3, true>;
@@ -106,8 +105,7 @@ ZHD int with_batch_plan(int L, int variant, F&& f) {
     case 6: case 7: return f(B3a{});
     case 8: return f(B4a{});
     case 9: return f(B4b{});
-    case 10: return f(B5a{});
-    case 11: return f(B5b{});
+    case 10: case 11: return f(B5b{});
     case 12: return f(B6a{});
     case 13: return f(B6b{});
     case 14: return f(B7a{});

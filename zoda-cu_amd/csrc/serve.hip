@@ -73,7 +73,6 @@ class Server {
   int init(uint64_t budget, uint32_t nslots) {
     ZODA_TRY(current_state(&st_));
     budget_ = budget;
-    ZODA_TRY(batch_grid(st_, &grid_));
     tabs_.assign((BATCH_MAX_LOG + 1) * BATCH_VARIANTS, EncodeTabs{});
     have_.reset(new bool[tabs_.size()]());
     ZODA_HIP(hipMalloc((void**)&d_tabs_, tabs_.size() * sizeof(EncodeTabs)));
@@ -260,10 +259,13 @@ class Server {
       return reqs_[x].sq.k > reqs_[y].sq.k;
     });
     uint64_t items = 0;
+    uint32_t lds_bytes = 4, max_log = 0;
     for (uint32_t q = 0; q < taken.size(); q++) {
       ZODA_TRY(describe_square(st_, reqs_[taken[q]].sq, &b.sq[q], tabs_.data(), have_.get()));
       b.first[q] = (uint32_t)items;
       items += b.sq[q].tiles;
+      lds_bytes = std::max(lds_bytes, batch_lds_bytes(b.sq[q].L, b.sq[q].variant));
+      max_log = std::max(max_log, b.sq[q].L);
     }
     if (items >= (1ull << 31)) return fail(ZODA_EINVAL, "batch of %llu tiles is too large", (unsigned long long)items);
     b.n = (uint32_t)taken.size();
@@ -273,8 +275,9 @@ class Server {
     b.counter = s.d_counter;
     b.tabs = d_tabs_;
     ZODA_TRY(sync_tables());
-    uint32_t advance = 0;
-    ZODA_TRY(launch_serve_batch(b, grid_, s.stream, &advance));
+    uint32_t advance = 0, grid = 0;
+    ZODA_TRY(batch_grid(st_, lds_bytes, max_log, &grid));
+    ZODA_TRY(launch_serve_batch(b, grid, lds_bytes, max_log, s.stream, &advance));
     s.base += advance;
     return commit(s, std::move(taken), bytes, launched);
   }
@@ -325,7 +328,7 @@ class Server {
 
   DeviceState* st_ = nullptr;
   uint64_t budget_ = 0, in_flight_bytes_ = 0, peak_bytes_ = 0;
-  uint32_t grid_ = 0, batches_ = 0, max_batch_ = 0, rejected_ = 0;
+  uint32_t batches_ = 0, max_batch_ = 0, rejected_ = 0;
   size_t big_in_flight_ = 0;
   uint64_t completed_ = 0;
   std::vector<EncodeTabs> tabs_;
