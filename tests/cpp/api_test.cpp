@@ -2,7 +2,7 @@
 // 150-237) and its encode validation (src/benchmarks/utils.rs:186-219),
 // restated against zoda/zoda.hpp, with the CPU oracle (oracle/liboracle.so)
 // as the checker.  Built by __graft_entry__.build(); run by
-// tests/test_gpu_cpp.py on a GPU box.  Prints one line per test and exits
+// tests/test_gpu_cpp.py on a machine with a GPU.  Prints one line per test and exits
 // non-zero on the first mismatch.
 #include <cstdio>
 #include <cstdlib>
