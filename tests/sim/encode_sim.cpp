@@ -47,25 +47,25 @@ static int sim_plan(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out
   return 0;
 }
 
-template <class PL, int STEP>
+template <class PL, bool BUF, int STEP>
 static void run_late_steps(uint32_t tile, const uint32_t* in, uint32_t* out, uint32_t cols,
                            int32_t* lds, const EncodeTabs& T) {
   for (int tid = 0; tid < PL::NT; tid++)
-    encode_step<PL, STEP, true>(tid, tile, in, out, (uint32_t)PL::S, cols, 0, lds, T);
-  if constexpr (STEP + 1 < PL::STEPS) run_late_steps<PL, STEP + 1>(tile, in, out, cols, lds, T);
+    encode_step<PL, STEP, true, BUF>(tid, tile, in, out, (uint32_t)PL::S, cols, 0, lds, T);
+  if constexpr (STEP + 1 < PL::STEPS) run_late_steps<PL, BUF, STEP + 1>(tile, in, out, cols, lds, T);
 }
 
 // The pipelined persistent kernel's order of work on one work-group
 // (encode_pipelined): the first pass from the registers loaded ahead with the
 // even-row stores of those registers, the middle passes with the loads of the
 // next tile into the same registers in parts, the last pass.
-template <class PL, int STEP>
+template <class PL, bool BUF, int STEP>
 static void run_step_loading(uint32_t tile, const uint32_t* in, uint32_t* out, uint32_t cols, int32_t* lds,
                              const EncodeTabs& T, bool more, uint32_t next, int part0, int np,
                              std::vector<uint32_t>* raw, int words) {
-  using PP = Pipe<PL>;
+  using PP = Pipe<PL, BUF>;
   for (int tid = 0; tid < PL::NT; tid++)
-    encode_step<PL, STEP, true>(tid, tile, in, out, (uint32_t)PL::S, cols, 0, lds, T, [&](int gi) {
+    encode_step<PL, STEP, true, BUF>(tid, tile, in, out, (uint32_t)PL::S, cols, 0, lds, T, [&](int gi) {
       if (more)
         PP::load_part(tid, next, in, cols,
                       *reinterpret_cast<uint32_t(*)[PP::G][PP::RR]>(raw->data() + (size_t)tid * words), part0 + gi,
@@ -73,9 +73,10 @@ static void run_step_loading(uint32_t tile, const uint32_t* in, uint32_t* out, u
     });
 }
 
-template <class PL>
+// BUF as the launch chooses it: raw-buffer accesses when the output is below 4 GiB.
+template <class PL, bool BUF>
 static int sim_pipelined(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out) {
-  using PP = Pipe<PL>;
+  using PP = Pipe<PL, BUF>;
   if (cols % PL::W != 0 || k != (uint32_t)PL::S) return -3;
   EncodeTabOffsets off;
   std::vector<Tw> tabs = build_encode_tables(PL::L, PL::R1, PL::R2, PL::RD, &off);
@@ -102,13 +103,13 @@ static int sim_pipelined(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t
     for (int tid = 0; tid < PL::NT; tid++)
       PP::first(tid, raw(tid), lds.data(), T, inv1,
                 [&](int gi) { PP::store_even(tid, gi, tile, out, cols, raw(tid)[gi]); });
-    run_step_loading<PL, 1>(tile, in, out, cols, lds.data(), T, more, tile + 1, 0, NP, &regs, WORDS);
-    run_step_loading<PL, 2>(tile, in, out, cols, lds.data(), T, more, tile + 1, G1, NP, &regs, WORDS);
-    run_step_loading<PL, 3>(tile, in, out, cols, lds.data(), T, more, tile + 1, G1 + G2, NP, &regs, WORDS);
+    run_step_loading<PL, BUF, 1>(tile, in, out, cols, lds.data(), T, more, tile + 1, 0, NP, &regs, WORDS);
+    run_step_loading<PL, BUF, 2>(tile, in, out, cols, lds.data(), T, more, tile + 1, G1, NP, &regs, WORDS);
+    run_step_loading<PL, BUF, 3>(tile, in, out, cols, lds.data(), T, more, tile + 1, G1 + G2, NP, &regs, WORDS);
     if constexpr (G4 > 0)
-      run_step_loading<PL, 4>(tile, in, out, cols, lds.data(), T, more, tile + 1, 2 * G1 + G2, NP, &regs, WORDS);
+      run_step_loading<PL, BUF, 4>(tile, in, out, cols, lds.data(), T, more, tile + 1, 2 * G1 + G2, NP, &regs, WORDS);
     else
-      run_late_steps<PL, 4>(tile, in, out, cols, lds.data(), T);
+      run_late_steps<PL, BUF, 4>(tile, in, out, cols, lds.data(), T);
   }
   return 0;
 }
@@ -120,7 +121,10 @@ extern "C" int zoda_sim_encode_persistent(const uint32_t* in, uint32_t k, uint32
   while (nk < k) { nk <<= 1; L++; }
   return with_encode_plan((int)L, [&](auto plan) -> int {
     using PL = decltype(plan);
-    if constexpr (PL::PERSIST) return sim_pipelined<PL>(in, k, cols, out);
+    if constexpr (PL::PERSIST) {
+      if (((uint64_t)(2 * PL::S - 1) * cols + PL::W) * 4 < (1ull << 32)) return sim_pipelined<PL, true>(in, k, cols, out);
+      return sim_pipelined<PL, false>(in, k, cols, out);
+    }
     else return -4;
   });
 }

@@ -55,13 +55,15 @@ __global__ __launch_bounds__(PL::NT) void encode_kernel(const uint32_t* __restri
 // even-row stores go out during the first pass, straight from the registers
 // it reads; the next tile's loads are issued in parts, one per group of the
 // middle passes (and, for plans with LOAD_LAST, of the last pass); the
-// odd-row stores leave the last pass as it produces them.
-template <class PL>
+// odd-row stores leave the last pass as it produces them.  BUF: every global
+// access goes through a raw buffer with 32-bit offsets from the tile's first
+// column (outputs below 4 GiB).
+template <class PL, bool BUF>
 __global__ __launch_bounds__(PL::NT) void encode_pipelined(const uint32_t* __restrict__ in,
                                                            uint32_t* __restrict__ out,
                                                            uint32_t cols, uint32_t ntiles,
                                                            EncodeTabs T) {
-  using PP = Pipe<PL>;
+  using PP = Pipe<PL, BUF>;
   // groups per thread of the passes between the first and the last
   constexpr int G1 = (PL::W << (PL::L - (PL::D == 2 ? PL::RD : PL::R2))) / PL::NT;
   constexpr int G2 = PL::D == 3 ? (PL::W << (PL::L - PL::RD)) / PL::NT : 0;
@@ -85,28 +87,28 @@ __global__ __launch_bounds__(PL::NT) void encode_pipelined(const uint32_t* __res
     const int tid = (int)threadIdx.x;
     PP::first(tid, raw, lds, T, inv1, [&](int gi) { PP::store_even(tid, gi, tile, out, cols, raw[gi]); });
     __syncthreads();
-    encode_step<PL, 1, true>(tid, tile, in, out, k, cols, 0, lds, T, [&](int gi) {
+    encode_step<PL, 1, true, BUF>(tid, tile, in, out, k, cols, 0, lds, T, [&](int gi) {
       if (more) PP::load_part(tid, ntile, in, cols, raw, gi, NP);
     });
     __syncthreads();
     if constexpr (PL::D == 3) {
-      encode_step<PL, 2, true>(tid, tile, in, out, k, cols, 0, lds, T, [&](int gi) {
+      encode_step<PL, 2, true, BUF>(tid, tile, in, out, k, cols, 0, lds, T, [&](int gi) {
         if (more) PP::load_part(tid, ntile, in, cols, raw, G1 + gi, NP);
       });
       __syncthreads();
-      encode_step<PL, 3, true>(tid, tile, in, out, k, cols, 0, lds, T, [&](int gi) {
+      encode_step<PL, 3, true, BUF>(tid, tile, in, out, k, cols, 0, lds, T, [&](int gi) {
         if (more) PP::load_part(tid, ntile, in, cols, raw, G1 + G2 + gi, NP);
       });
       __syncthreads();
       if constexpr (G4 > 0) {
-        encode_step<PL, 4, true>(tid, tile, in, out, k, cols, 0, lds, T, [&](int gi) {
+        encode_step<PL, 4, true, BUF>(tid, tile, in, out, k, cols, 0, lds, T, [&](int gi) {
           if (more) PP::load_part(tid, ntile, in, cols, raw, G1 + G2 + G3 + gi, NP);
         });
       } else {
-        encode_step<PL, 4, true>(tid, tile, in, out, k, cols, 0, lds, T);
+        encode_step<PL, 4, true, BUF>(tid, tile, in, out, k, cols, 0, lds, T);
       }
     } else {
-      encode_step<PL, 2, true>(tid, tile, in, out, k, cols, 0, lds, T);
+      encode_step<PL, 2, true, BUF>(tid, tile, in, out, k, cols, 0, lds, T);
     }
     __syncthreads();  // the next tile's first pass writes LDS
   }
@@ -179,9 +181,11 @@ static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32
   const bool full = cols % PL::W == 0 && k == (uint32_t)PL::S;
   const bool row = layout == ZODA_LAYOUT_ROW;
   if constexpr (PL::PERSIST) {
-    // Pipe addresses rows within a digit with 32-bit byte offsets.
+    // Pipe addresses rows within a digit with 32-bit byte offsets; whole
+    // arrays of at most 4 GiB also take the raw-buffer form.
     if (full && row && ((uint64_t)cols << (PL::LO1 + 1)) * 4 < (1ull << 32)) {
-      const void* pfn = (const void*)encode_pipelined<PL>;
+      const bool buf = ((uint64_t)(2 * PL::S - 1) * cols + PL::W) * 4 < (1ull << 32);
+      const void* pfn = buf ? (const void*)encode_pipelined<PL, true> : (const void*)encode_pipelined<PL, false>;
       uint32_t grid = 0;
       ZODA_TRY(kernel_setup(st, pfn, PL::NT, lds_bytes, true, &grid));
       if (dry) return 0;

@@ -100,7 +100,12 @@ struct EncodePlan {
   static constexpr int TILE_WORDS = D > 1 ? W * S : 0;
   // u32 words of the packed tables (see build_encode_tables)
   static constexpr int TW_WORDS = D == 1 ? S : D == 2 ? 2 * S : 3 * S + (1 << LO1);
-  static constexpr int LDS_WORDS = TILE_WORDS + (TWL ? TW_WORDS : 0);
+  // Three-digit plans with LDS tables also keep the middle pass's forward
+  // twiddles (2^LO1 entries) as (w, w/p) pairs behind the u32 tables, which
+  // saves that pass the conversion and the multiply of every twiddle read.
+  static constexpr bool MID_PAIRS = TWL && D == 3;
+  static constexpr int PAIR_WORDS = MID_PAIRS ? 4 << LO1 : 0;
+  static constexpr int LDS_WORDS = TILE_WORDS + (TWL ? TW_WORDS : 0) + PAIR_WORDS;
   static constexpr int STEPS = 2 * D - 1;
 };
 
@@ -168,6 +173,37 @@ struct NoHook {
   ZHD void operator()(int) const {}
 };
 
+// Raw-buffer view of a global u32 array.  An access names a per-thread 32-bit
+// byte offset and a uniform 32-bit byte offset, and the buffer instruction adds
+// both to the base itself, so the vector ALU computes no 64-bit address per
+// access (a global load or store costs one 64-bit add per value otherwise).
+// The hardware drops any access whose last byte lies at or beyond 2^32 - 1
+// bytes from the base (the range check of num_records = 2^32 - 1): callers keep
+// every sum of the two offsets at most 2^32 - 8.
+struct GBuf {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __amdgpu_buffer_rsrc_t r;
+  // dword 3 = 0x00020000: 32-bit data format, the gfx9 family's raw buffer
+  __device__ explicit GBuf(const uint32_t* p)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(p), 0, -1, 0x00020000)) {}
+  __device__ uint32_t load(uint32_t voff, uint32_t soff) const {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+  }
+  __device__ void store(uint32_t v, uint32_t voff, uint32_t soff) const {
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 0);
+  }
+#else
+  char* b;
+  explicit GBuf(const uint32_t* p) : b(reinterpret_cast<char*>(const_cast<uint32_t*>(p))) {}
+  uint32_t load(uint32_t voff, uint32_t soff) const {
+    return *reinterpret_cast<const uint32_t*>(b + ((size_t)voff + soff));
+  }
+  void store(uint32_t v, uint32_t voff, uint32_t soff) const {
+    *reinterpret_cast<uint32_t*>(b + ((size_t)voff + soff)) = v;
+  }
+#endif
+};
+
 // One pass for thread `tid` of the work-group encoding tile `tile`.
 //   LO, R    the digit this pass transforms
 //   KIND     PASS_INTT / PASS_MID / PASS_NTT
@@ -175,9 +211,12 @@ struct NoHook {
 //   LAST     write canonical odd rows to global memory
 //   LOA, RA  digit above, whose forward twiddle ends MID/NTT passes (RA = 0: none)
 //   FULL     whole tile and k = 2^L: no bounds checks
+//   BUF      LAST writes through a raw buffer based at the tile's first
+//            column (FULL tiles in the row layout with
+//            ((2 * 2^L - 1) * cols + W) * 4 < 2^32)
 //   hook     called with the group index before each group is transformed
 template <class PL, int LO, int R, int KIND, bool FIRST, bool LAST, int LOA, int RA, bool FULL,
-          class Hook>
+          bool BUF, class Hook>
 ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
                      uint32_t* __restrict__ out, uint32_t krows, uint32_t cols, int layout,
                      int32_t* __restrict__ lds, const EncodeTabs& T, const TwTab own,
@@ -277,11 +316,19 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
           for (int m = 0; m < RR; m++) {
             const uint32_t j = row_of<LO, R>(u, m);
             const uint32_t k = (j >> LOA) & ((1u << RA) - 1u);
-            x[m] = mulmod(x[m], tw_get<PL::TWL>(above, (k << LOA) | (j & ((1u << LOA) - 1u))));
+            constexpr bool U32 = PL::TWL && !(PL::MID_PAIRS && KIND == PASS_MID);
+            x[m] = mulmod(x[m], tw_get<U32>(above, (k << LOA) | (j & ((1u << LOA) - 1u))));
           }
         }
       }
-      if constexpr (LAST) {
+      if constexpr (LAST && BUF) {
+        static_assert(FULL, "buffer stores address whole tiles");
+        const GBuf ob(out + (size_t)tile * PL::W);
+        const uint32_t voff = ((2 * row_of<LO, R>(u, 0) + 1) * cols + c) * 4u;
+        const uint32_t ostep = (cols << (LO + 1)) * 4u;
+#pragma unroll
+        for (int m = 0; m < RR; m++) ob.store(canonical(x[m]), voff, (uint32_t)m * ostep);
+      } else if constexpr (LAST) {
         const uint32_t cidx = tile * PL::W + c;
         if (FULL || cidx < cols) {
           const size_t ostep = layout == 0 ? (size_t)cols << (LO + 1) : (size_t)2 << LO;
@@ -305,12 +352,16 @@ ZHD void encode_load_tables(int tid, int32_t* __restrict__ lds, const EncodeTabs
     uint32_t* dst = reinterpret_cast<uint32_t*>(lds + PL::TILE_WORDS);
     for (uint32_t i = (uint32_t)tid; i < T.packed_words; i += PL::NT) dst[i] = T.packed[i];
   }
+  if constexpr (PL::MID_PAIRS) {
+    Tw* pairs = reinterpret_cast<Tw*>(lds + PL::TILE_WORDS + PL::TW_WORDS);
+    for (uint32_t i = (uint32_t)tid; i < (1u << PL::LO1); i += PL::NT) pairs[i] = T.fwd2[i];
+  }
 }
 
 // Step STEP (0 .. 2D-2) of the encode of one tile; steps are separated by a
 // work-group barrier on the GPU.  FULL: every column of the tile exists and
 // k = 2^L, so no bounds checks are emitted.
-template <class PL, int STEP, bool FULL, class Hook = NoHook>
+template <class PL, int STEP, bool FULL, bool BUF = false, class Hook = NoHook>
 ZHD void encode_step(int tid, uint32_t tile, const uint32_t* __restrict__ in,
                      uint32_t* __restrict__ out, uint32_t krows, uint32_t cols, int layout,
                      int32_t* __restrict__ lds, const EncodeTabs& T,
@@ -321,36 +372,38 @@ ZHD void encode_step(int tid, uint32_t tile, const uint32_t* __restrict__ in,
   const TwTab inv1{T.inv1, PL::TWL ? sb + T.o_inv1 : nullptr};
   const TwTab inv2{T.inv2, PL::TWL ? sb + T.o_inv2 : nullptr};
   const TwTab fwd1{T.fwd1, PL::TWL ? sb + T.o_fwd1 : nullptr};
-  const TwTab fwd2{T.fwd2, PL::TWL ? sb + T.o_fwd2 : nullptr};
+  const Tw* mid_pairs =
+      PL::MID_PAIRS ? reinterpret_cast<const Tw*>(lds + PL::TILE_WORDS + PL::TW_WORDS) : T.fwd2;
+  const TwTab fwd2{mid_pairs, PL::TWL ? sb + T.o_fwd2 : nullptr};
   const TwTab twist{T.twist, PL::TWL ? sb + T.o_twist : nullptr};
   if constexpr (PL::D == 1) {
-    encode_pass<PL, 0, PL::RD, PASS_MID, true, true, 0, 0, FULL>(
+    encode_pass<PL, 0, PL::RD, PASS_MID, true, true, 0, 0, FULL, BUF>(
         tid, tile, in, out, krows, cols, layout, lds, T, none, none, twist, hook);
   } else if constexpr (PL::D == 2) {
     if constexpr (STEP == 0)
-      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL>(
+      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL, BUF>(
           tid, tile, in, out, krows, cols, layout, lds, T, inv1, none, none, hook);
     else if constexpr (STEP == 1)
-      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L1, PL::R1, FULL>(
+      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L1, PL::R1, FULL, BUF>(
           tid, tile, in, out, krows, cols, layout, lds, T, none, fwd1, twist, hook);
     else
-      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL>(
+      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL, BUF>(
           tid, tile, in, out, krows, cols, layout, lds, T, none, none, none, hook);
   } else {
     if constexpr (STEP == 0)
-      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL>(
+      encode_pass<PL, L1, PL::R1, PASS_INTT, true, false, 0, 0, FULL, BUF>(
           tid, tile, in, out, krows, cols, layout, lds, T, inv1, none, none, hook);
     else if constexpr (STEP == 1)
-      encode_pass<PL, L2, PL::R2, PASS_INTT, false, false, 0, 0, FULL>(
+      encode_pass<PL, L2, PL::R2, PASS_INTT, false, false, 0, 0, FULL, BUF>(
           tid, tile, in, out, krows, cols, layout, lds, T, inv2, none, none, hook);
     else if constexpr (STEP == 2)
-      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L2, PL::R2, FULL>(
+      encode_pass<PL, 0, PL::RD, PASS_MID, false, false, L2, PL::R2, FULL, BUF>(
           tid, tile, in, out, krows, cols, layout, lds, T, none, fwd2, twist, hook);
     else if constexpr (STEP == 3)
-      encode_pass<PL, L2, PL::R2, PASS_NTT, false, false, L1, PL::R1, FULL>(
+      encode_pass<PL, L2, PL::R2, PASS_NTT, false, false, L1, PL::R1, FULL, BUF>(
           tid, tile, in, out, krows, cols, layout, lds, T, none, fwd1, none, hook);
     else
-      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL>(
+      encode_pass<PL, L1, PL::R1, PASS_NTT, false, true, 0, 0, FULL, BUF>(
           tid, tile, in, out, krows, cols, layout, lds, T, none, none, none, hook);
   }
 }
@@ -361,8 +414,9 @@ ZHD void encode_step(int tid, uint32_t tile, const uint32_t* __restrict__ in,
 // being transformed, so the loads are in flight behind that work and never
 // queue up in one burst -- first() transforms them into the LDS tile, and
 // store_even() writes them to the even output rows, one group at a time as
-// first() reaches it.
-template <class PL>
+// first() reaches it.  BUF: every access goes through a raw buffer (GBuf) based
+// at the tile's first column, which needs ((2 * 2^L - 1) * cols + W) * 4 < 2^32.
+template <class PL, bool BUF = false>
 struct Pipe {
   static constexpr int R = PL::R1, RR = 1 << R, LO = PL::LO1;
   static constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
@@ -370,13 +424,18 @@ struct Pipe {
 
   // Byte offset of group gi's first value in a row-major array of `cols`
   // columns (`stride` rows per top-digit row: 1 for the input, 2 for the even
-  // output rows).  Rows of the top digit are 2^LO apart, so one uniform base
-  // per digit value plus this 32-bit offset addresses every value; the caller
-  // guarantees (cols << (LO + 1)) * 4 < 2^32.
+  // output rows), from base(array, tile).  Rows of the top digit are 2^LO
+  // apart, so one uniform base per digit value plus this 32-bit offset
+  // addresses every value; the caller guarantees (cols << (LO + 1)) * 4 < 2^32.
   static ZHD uint32_t offset(int tid, int gi, uint32_t tile, uint32_t cols, uint32_t stride) {
     const uint32_t g = (uint32_t)(gi * PL::NT + tid);
     const uint32_t c = g % PL::W, u = g / PL::W;  // u < 2^LO: the row within the digit
-    return (u * stride * cols + tile * PL::W + c) * 4u;
+    return (u * stride * cols + (BUF ? 0u : tile * PL::W) + c) * 4u;
+  }
+
+  template <class T>
+  static ZHD T* base(T* p, uint32_t tile) {
+    return BUF ? p + (size_t)tile * PL::W : p;
   }
 
   template <class T>
@@ -385,14 +444,22 @@ struct Pipe {
         typename std::conditional<std::is_const<T>::value, const char, char>::type*>(base) + byte_offset);
   }
 
+  // Value m of group gi (rows of the top digit are 2^LO apart).
+  static ZHD uint32_t load_one(const uint32_t* __restrict__ in, uint32_t cols, uint32_t off, int m) {
+    if constexpr (BUF) {
+      return GBuf(in).load(off, (uint32_t)m * ((cols << LO) * 4u));
+    } else {
+      return *at(in + (size_t)m * ((size_t)cols << LO), off);
+    }
+  }
+
   static ZHD void load(int tid, uint32_t tile, const uint32_t* __restrict__ in, uint32_t cols,
                        uint32_t (&raw)[G][RR]) {
-    const size_t istep = (size_t)cols << LO;
 #pragma unroll
     for (int gi = 0; gi < G; gi++) {
       const uint32_t off = offset(tid, gi, tile, cols, 1);
 #pragma unroll
-      for (int m = 0; m < RR; m++) raw[gi][m] = *at(in + (size_t)m * istep, off);
+      for (int m = 0; m < RR; m++) raw[gi][m] = load_one(base(in, tile), cols, off, m);
     }
   }
 
@@ -402,22 +469,28 @@ struct Pipe {
   // constant and the conditions fold away.
   static ZHD void load_part(int tid, uint32_t tile, const uint32_t* __restrict__ in, uint32_t cols,
                             uint32_t (&raw)[G][RR], int p, int np) {
-    const size_t istep = (size_t)cols << LO;
 #pragma unroll
     for (int gi = 0; gi < G; gi++) {
       const uint32_t off = offset(tid, gi, tile, cols, 1);
 #pragma unroll
       for (int m = 0; m < RR; m++)
-        if ((gi * RR + m) * np / (G * RR) == p) raw[gi][m] = *at(in + (size_t)m * istep, off);
+        if ((gi * RR + m) * np / (G * RR) == p) raw[gi][m] = load_one(base(in, tile), cols, off, m);
     }
   }
 
   static ZHD void store_even(int tid, int gi, uint32_t tile, uint32_t* __restrict__ out,
                              uint32_t cols, const uint32_t (&r)[RR]) {
-    const size_t ostep = (size_t)cols << (LO + 1);
     const uint32_t off = offset(tid, gi, tile, cols, 2);
+    if constexpr (BUF) {
+      const GBuf ob(base(out, tile));
+      const uint32_t ostep = (cols << (LO + 1)) * 4u;
 #pragma unroll
-    for (int m = 0; m < RR; m++) *at(out + (size_t)m * ostep, off) = r[m];
+      for (int m = 0; m < RR; m++) ob.store(r[m], off, (uint32_t)m * ostep);
+    } else {
+      const size_t ostep = (size_t)cols << (LO + 1);
+#pragma unroll
+      for (int m = 0; m < RR; m++) *at(out + (size_t)m * ostep, off) = r[m];
+    }
   }
 
   // The first pass (INTT of the top digit) from registers into the LDS tile;

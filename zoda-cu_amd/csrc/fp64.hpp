@@ -54,17 +54,15 @@ ZHD double reduce(double x) {
 }
 
 // |x| < 2^36 (an integer) -> canonical value in [0, p), by floor division:
-//   y = x/p - 1/2 + 2^-40   one FMA; with |x/p| < 2^6 the error of 1/p adds
-//                           below 2^-47 and the rounding of y at most 2^-47
-//   q = round(y)            = floor(x/p): x/p = n + j/p with 0 <= j < p, so
-//                           y is within 2^-46 of a value between
-//                           n - 1/2 + 2^-40 and n + 1/2 - 1/p + 2^-40, and
-//                           1/p > 2^-31
+//   y = x/p + 2^-40         one FMA; with |x/p| < 2^6 the error of 1/p adds
+//                           below 2^-47 and the rounding of y at most 2^-48
+//   q = floor(y)            = floor(x/p): x/p = n + j/p with 0 <= j < p, so
+//                           y is within 2^-46 of a value between n + 2^-40 and
+//                           n + 1 - 1/p + 2^-40, and 1/p > 2^-31
 //   r = x - q p             in [0, p), exact.
-constexpr double FLOOR_BIAS = -0.5 + 0x1p-40;
+constexpr double FLOOR_EPS = 0x1p-40;
 ZHD uint32_t canonical(double x) {
-  const double y = fmad(x, PINV, FLOOR_BIAS);
-  const double q = (y + ROUND_MAGIC) - ROUND_MAGIC;
+  const double q = __builtin_floor(fmad(x, PINV, FLOOR_EPS));
   return (uint32_t)fmad(-q, PD, x);
 }
 
