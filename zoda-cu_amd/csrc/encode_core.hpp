@@ -189,8 +189,11 @@ struct GBuf {
   __device__ uint32_t load(uint32_t voff, uint32_t soff) const {
     return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
   }
+  // Stores are non-temporal (cache policy bit nt): no kernel here reads the
+  // output again.  Measured against plain stores on one box: 1-2 % faster at
+  // side 2048, 0.5-4 % at 1024, 7-16 % at 512.
   __device__ void store(uint32_t v, uint32_t voff, uint32_t soff) const {
-    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 2);
   }
 #else
   char* b;
