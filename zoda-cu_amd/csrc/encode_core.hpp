@@ -168,6 +168,23 @@ ZHD size_t out_index(uint32_t row, uint32_t cidx, uint32_t cols, uint32_t out_ro
   return layout == 0 ? (size_t)row * cols + cidx : (size_t)cidx * out_rows + row;
 }
 
+// One output value.  In the row layout, plans whose row segments are whole
+// 128-byte lines (W >= 32) store non-temporally (measured against plain
+// stores: 6-8 % at sides 128 and 256, 5 % over the batched mixed-side queue);
+// narrower tiles write parts of lines that neighbouring tiles complete.
+template <class PL>
+ZHD void store_out(uint32_t* p, uint32_t v, int layout) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (PL::W >= 32) {
+    if (layout == 0) {
+      __builtin_nontemporal_store(v, p);
+      return;
+    }
+  }
+#endif
+  *p = v;
+}
+
 // Called with the group index at the start of every group of a pass.
 struct NoHook {
   ZHD void operator()(int) const {}
@@ -265,7 +282,7 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
         if (FULL || cidx < cols) {
           uint32_t* dst = out + out_index(2 * row_of<LO, R>(u, 0), cidx, cols, OUT_ROWS, layout);
 #pragma unroll
-          for (int m = 0; m < RR; m++) dst[(size_t)m * ostep] = raw[gi][m];
+          for (int m = 0; m < RR; m++) store_out<PL>(dst + (size_t)m * ostep, raw[gi][m], layout);
         }
       }
     }
@@ -337,7 +354,7 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
           const size_t ostep = layout == 0 ? (size_t)cols << (LO + 1) : (size_t)2 << LO;
           uint32_t* dst = out + out_index(2 * row_of<LO, R>(u, 0) + 1, cidx, cols, OUT_ROWS, layout);
 #pragma unroll
-          for (int m = 0; m < RR; m++) dst[(size_t)m * ostep] = canonical(x[m]);
+          for (int m = 0; m < RR; m++) store_out<PL>(dst + (size_t)m * ostep, canonical(x[m]), layout);
         }
       } else {
 #pragma unroll
