@@ -26,6 +26,7 @@ sys.path.insert(0, os.path.join(ROOT, "zoda-cu_amd"))
 METRIC = "encode Gb/s + %HBM roofline (2048² square); batched vs plain-loop p50/p99"
 PEAK_HBM_GBS = 8000.0
 SHARE_BYTES = 512
+P = 2013265921  # BabyBear
 
 
 def parse():
@@ -35,6 +36,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--side", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=2048)
+    ap.add_argument("--input", choices=("uniform", "ramp"), default="uniform",
+                    help="encode input: seeded uniform BabyBear, or the reference benchmark's ramp "
+                         "(row*cols+col) mod p (src/benchmarks/utils.rs:81-84)")
     ap.add_argument("--no-e2e", action="store_true", help="skip the pinned-host end-to-end timing")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-cols", type=int, default=16384, help="columns in the CPU sample")
@@ -337,7 +341,14 @@ def main():
     dev = torch.device("cuda:0")
     t_in = torch.empty((side, cols), dtype=torch.int32, device=dev)
     t_out = torch.empty((rows_out, cols), dtype=torch.int32, device=dev)
-    zoda_hip.fill_uniform(t_in, seed=args.seed)
+    if args.input == "ramp":
+        for r0 in range(0, side, 256):  # (row * cols + col) mod p, built on the device in slabs
+            r1 = min(side, r0 + 256)
+            idx = torch.arange(r0 * cols, r1 * cols, device=dev, dtype=torch.int64)
+            t_in[r0:r1].copy_((idx % P).to(torch.int32).view(r1 - r0, cols))
+            del idx
+    else:
+        zoda_hip.fill_uniform(t_in, seed=args.seed)
     stream = torch.cuda.current_stream(dev)
     sptr = stream.cuda_stream
 
@@ -392,7 +403,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
-        "data": "synthetic: seeded uniform BabyBear u32 (zoda_hip_fill_uniform), device-resident",
+        "data": ("synthetic: the reference benchmark's ramp (row*cols+col) mod p, device-resident"
+                 if args.input == "ramp" else
+                 "synthetic: seeded uniform BabyBear u32 (zoda_hip_fill_uniform), device-resident"),
         "config": {
             "workload": f"plain ZODA encode of a {side}x{side} square of 512 B shares, device-resident",
             "side": side, "rows_in": side, "cols": cols, "rows_out": rows_out,

@@ -124,11 +124,17 @@ ZHD void dec_intt(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
 }
 
 // Middle pass on the bottom digit: inverse DFT, the per-value twist factor,
-// forward DFT, forward twiddle of the digit above (LDS table `ta`).
-template <class PL>
+// forward DFT, forward twiddle of the digit above (LDS table `ta`).  The
+// forward DFT's output index is the row's low RD bits.  Second pair (DATA):
+// the twist is 0 for m >= 2^(RD-1) and only even rows are kept, and the even
+// outputs of a DFT whose upper half is zero are the DFT of half the size of
+// the lower half, X[2j] = sum_{m < H} z_m (w^2)^(jm); so the pass runs that
+// and writes the even rows only.
+template <class PL, bool DATA>
 ZHD void dec_mid(int tid, int32_t* __restrict__ lds, const Tw* __restrict__ mid, const uint32_t* __restrict__ ta,
                  const Tw* __restrict__ roots) {
   constexpr int R = PL::RD, RR = 1 << R, LOA = PL::LO2, RA = PL::R2;
+  constexpr int RO = DATA ? R - 1 : R, RO_N = 1 << RO;  // forward DFT size
   constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
   static_assert((PL::W << (PL::L - R)) % PL::NT == 0, "groups must divide evenly over the threads");
   const TwTab above{nullptr, ta};
@@ -141,31 +147,37 @@ ZHD void dec_mid(int tid, int32_t* __restrict__ lds, const Tw* __restrict__ mid,
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<0, R>(u, m), c)];
     dft_reg<R>(x, roots + 64);
+    double y[RO_N];
+    y[0] = x[0];  // mid[0] = 1
 #pragma unroll
-    for (int m = 1; m < RR; m++) x[m] = mulmod(x[m], mid[m]);  // mid[0] = 1
-    dft_reg<R>(x, roots);
+    for (int m = 1; m < RO_N; m++) y[m] = mulmod(x[m], mid[m]);
+    dft_reg<RO>(y, roots);
 #pragma unroll
-    for (int m = 0; m < RR; m++) {
-      const uint32_t j = row_of<0, R>(u, m);
+    for (int q = 0; q < RO_N; q++) {
+      const int m = DATA ? 2 * q : q;  // the row's low RD bits
+      const uint32_t j = row_of<0, R>(u, (uint32_t)m);
       const uint32_t k = (j >> LOA) & ((1u << RA) - 1u);
-      x[m] = mulmod(x[m], tw_get<true>(above, (k << LOA) | (j & ((1u << LOA) - 1u))));
+      y[q] = mulmod(y[q], tw_get<true>(above, (k << LOA) | (j & ((1u << LOA) - 1u))));
+      lds[lds_at<PL>(j, c)] = (int32_t)y[q];
     }
-#pragma unroll
-    for (int m = 0; m < RR; m++) lds[lds_at<PL>(row_of<0, R>(u, m), c)] = (int32_t)x[m];
   }
 }
 
 // NTT pass on the second digit: forward DFT, forward twiddle of the top digit.
-template <class PL>
+// Second pair (DATA): only the groups of even rows (the row's low bit is the
+// group's).
+template <class PL, bool DATA>
 ZHD void dec_ntt2(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ ta, const Tw* __restrict__ roots) {
   constexpr int R = PL::R2, RR = 1 << R, LO = PL::LO2, LOA = PL::LO1, RA = PL::R1;
-  constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
-  static_assert((PL::W << (PL::L - R)) % PL::NT == 0, "groups must divide evenly over the threads");
+  constexpr int NG = (PL::W << (PL::L - R)) >> (DATA ? 1 : 0);
+  static_assert(NG % PL::NT == 0 || PL::NT % NG == 0, "groups must divide evenly over the threads");
+  constexpr int G = NG >= PL::NT ? NG / PL::NT : 1;
 #pragma unroll
   for (int gi = 0; gi < G; gi++) {
     group_fence();
     const uint32_t g = (uint32_t)(gi * PL::NT + tid);
-    const uint32_t c = g % PL::W, u = g / PL::W;
+    if (NG < PL::NT && g >= (uint32_t)NG) break;  // fewer groups than threads
+    const uint32_t c = g % PL::W, u = (g / PL::W) << (DATA ? 1 : 0);
     double x[RR];
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<LO, R>(u, m), c)];
@@ -261,9 +273,9 @@ ZHD void dec_step(int tid, int32_t* __restrict__ lds, const DecodeTabs& T, const
   else if constexpr (K == 1)
     dec_intt<PL, PL::LO2, PL::R2, true>(tid, lds, sb + T.o_inv2, sb + T.o_fold[P], T.roots);
   else if constexpr (K == 2)
-    dec_mid<PL>(tid, lds, T.mid[P], sb + T.o_fwd2, T.roots);
+    dec_mid<PL, P == 1>(tid, lds, T.mid[P], sb + T.o_fwd2, T.roots);
   else if constexpr (K == 3)
-    dec_ntt2<PL>(tid, lds, sb + T.o_fwd1, T.roots);
+    dec_ntt2<PL, P == 1>(tid, lds, sb + T.o_fwd1, T.roots);
   else
     dec_ntt1<PL, P == 1>(tid, lds, zinv, T.roots);
 }
