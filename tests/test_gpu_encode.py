@@ -188,3 +188,27 @@ def test_big_square_2048(zh, torch):
 
 def test_big_square_4096(zh, torch):
     big_square_checks(zh, torch, 4096, seed=4096)
+
+
+
+@pytest.mark.parametrize("side,cols", [(2048, 262208), (2048, 262224), (1024, 524512)])
+def test_raw_buffer_offset_limit(zh, torch, side, cols):
+    # Whole tiles of the persistent kernel around the raw-buffer form's limit
+    # (((2 side - 1) cols + W) * 4 < 2^32): 2048 x 262208 and 1024 x 524512
+    # are inside it (the last output word's offset from its tile's base is
+    # 196 bytes and 257 KiB below 2^32); 2048 x 262224 is past it and takes
+    # 64-bit pointers.
+    t_in = torch.empty((side, cols), dtype=torch.int32, device="cuda")
+    zh.fill_uniform(t_in, seed=cols)
+    print_case("wide-encode", side=side, cols=cols)
+    t_out = zh.encode_tensor(t_in)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(cols)
+    cols_idx = sorted(set(rng.choice(cols, size=16, replace=False).tolist()) | {0, cols - 1})
+    idx = torch.tensor(cols_idx, device="cuda")
+    data_s = to_host(t_in[:, idx].contiguous())
+    got_s = to_host(t_out[:, idx].contiguous())
+    assert np.array_equal(got_s, O.encode(data_s, threads=16))
+    assert torch.equal(t_out[0::2], t_in)
+    del t_in, t_out
+    torch.cuda.empty_cache()
