@@ -38,6 +38,8 @@ constexpr uint32_t DEC_COSET = 31;  // a generator: h w^r is never a 2k-th root 
 constexpr uint32_t DEC_MAX_K = 16384;  // n = 2k values of one column in LDS: <= 128 KiB
 constexpr int TT = 32;              // transpose tile
 constexpr int DEC_NT = 256;
+constexpr int GATHER_NY = 2;  // dec_gather_ranked blocks are TT x GATHER_NY threads
+constexpr int SCATTER_NY = 2;  // dec_scatter blocks are TT x SCATTER_NY threads
 
 constexpr uint32_t MONT_ONE = (uint32_t)((1ull << 32) % P);  // 1 in Montgomery form
 
@@ -109,24 +111,32 @@ __global__ void dec_gather(const uint32_t* __restrict__ enc, uint32_t cols, uint
 
 // The known rows only, packed by rank: T[c * k + i] = enc[rows[i] * cols + c]
 // * Z(w^rows[i]) for the k known rows rows[0..k) (a 32 x 32 tile of (rank,
-// column) per block: whole 128-byte segments both ways).
+// column) per block: whole 128-byte segments both ways).  The block's row
+// numbers and factors are read once into LDS, so every value costs one global
+// load, and each thread moves 32 / GATHER_NY values (at 2048^2: 1.73 ms with
+// per-value row and factor loads and 4 values per thread, 1.09 ms so).
 __global__ void dec_gather_ranked(const uint32_t* __restrict__ enc, uint32_t cols, uint32_t k,
                                   const uint32_t* __restrict__ rows, const uint32_t* __restrict__ zk_m,
                                   uint32_t* __restrict__ T) {
   __shared__ uint32_t tile[TT][TT + 1];
+  __shared__ uint32_t rid[TT], zr[TT];  // the block's 32 row numbers and their Z(w^r)
   const uint32_t c0 = blockIdx.x * TT, i0 = blockIdx.y * TT;
-  const uint32_t tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
-  for (uint32_t q = ty; q < TT; q += 8) {
+  const uint32_t tx = threadIdx.x, ty = threadIdx.y;  // 32 x GATHER_NY
+  if (ty == 0) {
+    const uint32_t i = i0 + tx;
+    const uint32_t r = i < k ? rows[i] : 0u;
+    rid[tx] = r;
+    zr[tx] = i < k ? zk_m[r] : 0u;
+  }
+  __syncthreads();
+  for (uint32_t q = ty; q < TT; q += GATHER_NY) {
     const uint32_t i = i0 + q, c = c0 + tx;
     uint32_t v = 0;
-    if (i < k && c < cols) {
-      const uint32_t r = rows[i];
-      v = mont_mul(enc[(size_t)r * cols + c], zk_m[r]);
-    }
+    if (i < k && c < cols) v = mont_mul(enc[(size_t)rid[q] * cols + c], zr[q]);
     tile[q][tx] = v;
   }
   __syncthreads();
-  for (uint32_t q = ty; q < TT; q += 8) {
+  for (uint32_t q = ty; q < TT; q += GATHER_NY) {
     const uint32_t c = c0 + q, i = i0 + tx;
     if (c < cols && i < k) T[(size_t)c * k + i] = tile[tx][q];
   }
@@ -297,12 +307,12 @@ __global__ void dec_scatter(const uint32_t* __restrict__ T, uint32_t stride, uin
   __shared__ uint32_t tile[TT][TT + 1];
   const uint32_t j0 = blockIdx.x * TT, c0 = blockIdx.y * TT;
   const uint32_t tx = threadIdx.x, ty = threadIdx.y;
-  for (uint32_t i = ty; i < TT; i += 8) {
+  for (uint32_t i = ty; i < TT; i += SCATTER_NY) {
     const uint32_t c = c0 + i, j = j0 + tx;
     tile[i][tx] = (c < cols && j < k) ? T[(size_t)c * stride + j] : 0u;
   }
   __syncthreads();
-  for (uint32_t i = ty; i < TT; i += 8) {
+  for (uint32_t i = ty; i < TT; i += SCATTER_NY) {
     const uint32_t j = j0 + i, c = c0 + tx;
     if (j < k && c < cols) data[(size_t)j * cols + c] = tile[tx][i];
   }
@@ -412,7 +422,7 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
     hipLaunchKernelGGL(dec_prep, dim3((2 * n + 255) / 256), dim3(256), 0, s, d_x, d_known, d_ep, ne, log_n,
                        to_mont_host(DEC_COSET), fp64 ? 1 : 0, d_zk, d_zinv);
     if (fp64) {
-      hipLaunchKernelGGL(dec_gather_ranked, dim3((cols + TT - 1) / TT, (k + TT - 1) / TT), dim3(TT, 8), 0, s, d_enc,
+      hipLaunchKernelGGL(dec_gather_ranked, dim3((cols + TT - 1) / TT, (k + TT - 1) / TT), dim3(TT, GATHER_NY), 0, s, d_enc,
                          cols, k, d_rows, d_zk, T);
       (void)with_decode_plan((int)log_n, [&](auto plan) -> int {
         using PL = decltype(plan);
@@ -427,7 +437,7 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
       hipLaunchKernelGGL(dec_fused, dim3(cols), dim3(DEC_NT), lds, s, T, log_n, tn.fwd, tn.inv, tk.fwd, d_hpow,
                          d_zinv, d_hinv);
     }
-    hipLaunchKernelGGL(dec_scatter, dim3((k + TT - 1) / TT, (cols + TT - 1) / TT), dim3(TT, 8), 0, s, T, t_rows, k,
+    hipLaunchKernelGGL(dec_scatter, dim3((k + TT - 1) / TT, (cols + TT - 1) / TT), dim3(TT, SCATTER_NY), 0, s, T, t_rows, k,
                        cols, d_data);
     if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "decode launch failed");
   }
