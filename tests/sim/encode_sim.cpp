@@ -170,17 +170,18 @@ extern "C" void zoda_sim_rlc_coefficient(const uint8_t* root, uint32_t len, uint
 // ---- erasure decoder (decode_core.hpp) ------------------------------------------
 
 template <class PL, int STEP>
-static void dec_run_steps(int32_t* lds, const DecodeTabs& T, const uint32_t* zinv) {
-  for (int tid = 0; tid < PL::NT; tid++) dec_step<PL, STEP>(tid, lds, T, zinv);
-  if constexpr (STEP < 9) dec_run_steps<PL, STEP + 1>(lds, T, zinv);
+static void dec_run_steps(int32_t* lds, const DecodeTabs& T, const uint32_t* srow) {
+  for (int tid = 0; tid < PL::NT; tid++) dec_step<PL, STEP>(tid, lds, T, srow);
+  if constexpr (STEP < 4) dec_run_steps<PL, STEP + 1>(lds, T, srow);
 }
 
 // The decoder's column kernel on the host: `enc` is the 2k x cols extended
 // square (row-major), the first k distinct of `known_rows` are used, `out`
-// receives the k x cols data.  The vanishing-polynomial values are computed
-// here directly from their definition; the scratch holds the known rows by
-// rank, as the GPU's gather leaves it.  Returns -1 for sides without an FP64
-// plan.
+// receives the k x cols data.  The vanishing polynomial's values and the
+// erased rows' factors are computed here directly from their definitions;
+// the scratch holds the known rows by rank, as the GPU's gather leaves it,
+// and the known data rows are copied as the gather copies them.  Returns -1
+// for sides without an FP64 plan.
 extern "C" int zoda_sim_decode(const uint32_t* enc, uint32_t k, uint32_t cols, const uint32_t* known_rows,
                                uint32_t nknown, uint32_t* out) {
   const uint32_t n = 2 * k;
@@ -195,7 +196,7 @@ extern "C" int zoda_sim_decode(const uint32_t* enc, uint32_t k, uint32_t cols, c
       have++;
     }
   if (have < k) return -2;
-  const uint32_t w = root_of_unity(L), h = 31;
+  const uint32_t w = root_of_unity(L);
   std::vector<uint32_t> x(n), erased;
   for (uint32_t r = 0, cur = 1; r < n; r++, cur = mul_mod_host(cur, w)) {
     x[r] = cur;
@@ -206,10 +207,18 @@ extern "C" int zoda_sim_decode(const uint32_t* enc, uint32_t k, uint32_t cols, c
     for (uint32_t e : erased) acc = mul_mod_host(acc, (pt + P - e) % P);
     return acc;
   };
-  std::vector<uint32_t> zk(n, 0), zinv(n);
+  // srow[e] = 1 / (n w^e Z'(w^e)) on the erased even rows, Z'(w^e) the product
+  // over the other erasures of (w^e - w^e'); 0 elsewhere.
+  std::vector<uint32_t> zk(n, 0), srow(n, 0);
   for (uint32_t r = 0; r < n; r++) {
-    if (known[r]) zk[r] = Z(x[r]);
-    zinv[r] = inv_mod_host(Z(mul_mod_host(h, x[r])));
+    if (known[r]) {
+      zk[r] = Z(x[r]);
+    } else if (r % 2 == 0) {
+      uint32_t acc = mul_mod_host(n, x[r]);
+      for (uint32_t e : erased)
+        if (e != x[r]) acc = mul_mod_host(acc, (x[r] + P - e) % P);
+      srow[r] = inv_mod_host(acc);
+    }
   }
   // The rank-packed scratch: per column the k known rows (in increasing
   // order); `rows` lists them, then the erased rows.
@@ -224,22 +233,23 @@ extern "C" int zoda_sim_decode(const uint32_t* enc, uint32_t k, uint32_t cols, c
       T[(size_t)c * k + i] = mul_mod_host(enc[(size_t)rows[i] * cols + c], zk[rows[i]]);
   const int rc = with_decode_plan((int)L, [&](auto plan) -> int {
     using PL = decltype(plan);
-    const DecodeTableImage im = build_decode_tables<PL>(h);
+    const DecodeTableImage im = build_decode_tables<PL>();
     if (im.packed.empty()) return -3;
-    static const DecodeConst uniform = make_decode_const(31);  // the kernel's constant-memory tables
-    const DecodeTabs tabs = make_decode_tabs(PL::L, im, im.packed.data(), uniform);
+    static const DecodeConst uniform = make_decode_const();  // the kernel's constant-memory roots
+    const DecodeTabs tabs = make_decode_tabs(im, im.packed.data(), uniform);
     std::vector<int32_t> lds(PL::LDS_WORDS);
     for (int tid = 0; tid < PL::NT; tid++) dec_load_tables<PL>(tid, lds.data(), tabs);
     const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
     for (uint32_t tile = 0; tile < ntiles; tile++) {
       for (int tid = 0; tid < PL::NT; tid++) dec_load<PL>(tid, tile, cols, T.data(), rows.data(), lds.data());
-      dec_run_steps<PL, 0>(lds.data(), tabs, zinv.data());
+      dec_run_steps<PL, 0>(lds.data(), tabs, srow.data());
       for (int tid = 0; tid < PL::NT; tid++) dec_store<PL>(tid, tile, cols, T.data(), lds.data());
     }
     return 0;
   });
   if (rc != 0) return rc;
   for (uint32_t j = 0; j < k; j++)
-    for (uint32_t c = 0; c < cols; c++) out[(size_t)j * cols + c] = T[(size_t)c * k + j];
+    for (uint32_t c = 0; c < cols; c++)
+      out[(size_t)j * cols + c] = known[2 * j] ? enc[(size_t)2 * j * cols + c] : T[(size_t)c * k + j];
   return 0;
 }

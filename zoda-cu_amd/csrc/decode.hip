@@ -7,23 +7,33 @@
 // Per column, with K the k known rows and E the others (k of them), n = 2k:
 //   Z(x) = prod_{e in E} (x - w^e)                  vanishes on the erasures
 //   y_r  = out[r] * Z(w^r) on K, 0 on E              = g(w^r), g = f * Z, deg g < n
-//   G    = INTT_n(y)                                 coefficients of g
-//   g(h w^r) = NTT_n(G_i h^i)                        on a coset where Z != 0 (h = 31)
-//   f(h w^r) = g(h w^r) / Z(h w^r)
+//
+// For 2^8 <= n <= 2^14 (dec_columns, FP64, decode_core.hpp), from the formal
+// derivative: g' = f' Z + f Z', and Z(w^e) = 0, so at an erased row
+//   f(w^e) = w^e g'(w^e) / (w^e Z'(w^e)),   w^r g'(w^r) = NTT_n(i * G_i)_r,  G = INTT_n(y)
+// -- one transform pair, of which only the even rows are computed; the known
+// data rows are copies of the input.  For other n (dec_fused), on a coset
+// where Z has no zero (h = 31):
+//   g(h w^r) = NTT_n(G_i h^i),  f(h w^r) = g(h w^r) / Z(h w^r)
 //   f_i  = INTT_n(f(h w^r))_i * h^-i, i < k          (the upper k are zero)
 //   data[j] = f(w^(2j)) = NTT_k(f)_j
 //
 // Device work, all in canonical u32 with Montgomery multiplications:
-//   dec_prep    Z(w^r) on K and 1/Z(h w^r) for every r (n threads, k factors
+//   dec_prep    Z(w^r) on K, and 1/(n w^e Z'(w^e)) on the erased even rows
+//               (FP64 path) or 1/Z(h w^r) for every r (n threads, k factors
 //               each): the only O(k^2) part, once per call.
-//   dec_gather  row-major square -> column-major scratch, times Z(w^r)
-//               (tiled transpose; erased rows are never read).
+//   dec_gather_ranked  (FP64 path) the known rows packed by rank into a
+//               column-major scratch, times Z(w^r); the known even rows also
+//               straight into the data.
+//   dec_gather  (other n) row-major square -> full-height column-major
+//               scratch, times Z(w^r) (erased rows are never read).
 //   dec_fused   one work-group per column, in LDS: INTT_n (decimation in time,
 //               bit-reversed in, natural out), * h^i/n, NTT_n (decimation in
 //               frequency, natural in, bit-reversed out), * 1/Z(h w^r) in the
 //               same bit-reversed order, INTT_n, * h^-i/n, NTT_k -- no
 //               permutation passes, one read and one write of the scratch.
-//   dec_scatter column-major -> row-major data (tiled transpose).
+//   dec_scatter column-major -> row-major data (tiled transpose; on the FP64
+//               path only the erased even rows).
 #include <algorithm>
 #include <vector>
 
@@ -59,30 +69,45 @@ __device__ uint32_t mont_pow(uint32_t a, uint32_t e) {
 }
 
 // x_m[r] = w^r in Montgomery form; ep_m: the erased points, Montgomery form.
-// zk_m[r] = Z(w^r) (0 on erased rows).  With `natural`, zinv[i] = 1 / Z(h w^i)
-// as a canonical value; otherwise zinv[i] = 1 / Z(h w^(brev(i))) in
-// Montgomery form.
+// zk_m[r] = Z(w^r) (0 on erased rows).  FP64 path (`deriv`): srow[r] =
+// 1 / (n w^r Z'(w^r)) on the erased even rows, Z'(w^e) = prod over the other
+// erasures of (w^e - w^e'), and 0 on the other rows (canonical, n_m = n in
+// Montgomery form).  Otherwise srow[i] = 1 / Z(h w^(brev(i))) in Montgomery
+// form (the coset path).
 __global__ void dec_prep(const uint32_t* __restrict__ x_m, const uint8_t* __restrict__ known,
                          const uint32_t* __restrict__ ep_m, uint32_t ne, uint32_t log_n, uint32_t h_m,
-                         int natural, uint32_t* __restrict__ zk_m, uint32_t* __restrict__ zinv) {
+                         int deriv, uint32_t n_m, uint32_t* __restrict__ zk_m, uint32_t* __restrict__ srow) {
   const uint32_t n = 1u << log_n;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 2 * n) return;
-  const bool coset = t >= n;
-  const uint32_t r = coset ? (natural ? t - n : brev_n(t - n, log_n)) : t;
-  if (!coset && !known[r]) {
-    zk_m[r] = 0;
+  if (t < n) {  // Z on the known rows
+    if (!known[t]) {
+      zk_m[t] = 0;
+      return;
+    }
+    uint32_t acc = MONT_ONE;
+    for (uint32_t j = 0; j < ne; j++) acc = mont_mul(acc, sub_mod(x_m[t], ep_m[j]));
+    zk_m[t] = acc;
     return;
   }
-  const uint32_t x = coset ? mont_mul(h_m, x_m[r]) : x_m[r];
+  if (deriv) {
+    const uint32_t r = t - n;
+    if (known[r] || (r & 1u)) {
+      srow[r] = 0;
+      return;
+    }
+    const uint32_t x = x_m[r];
+    uint32_t acc = mont_mul(n_m, x);  // n w^r
+    for (uint32_t j = 0; j < ne; j++)
+      if (ep_m[j] != x) acc = mont_mul(acc, sub_mod(x, ep_m[j]));
+    srow[r] = mont_mul(mont_pow(acc, P - 2), 1u);  // mont_mul by 1 leaves Montgomery form
+    return;
+  }
+  const uint32_t r = brev_n(t - n, log_n);
+  const uint32_t x = mont_mul(h_m, x_m[r]);
   uint32_t acc = MONT_ONE;
   for (uint32_t j = 0; j < ne; j++) acc = mont_mul(acc, sub_mod(x, ep_m[j]));
-  if (coset) {
-    const uint32_t inv = mont_pow(acc, P - 2);
-    zinv[t - n] = natural ? mont_mul(inv, 1u) : inv;  // mont_mul by 1 leaves Montgomery form
-  } else {
-    zk_m[r] = acc;
-  }
+  srow[t - n] = mont_pow(acc, P - 2);
 }
 
 // T[c * n + r] = enc[r * cols + c] * Z(w^r) (a 32 x 32 tile per block).  The
@@ -111,13 +136,14 @@ __global__ void dec_gather(const uint32_t* __restrict__ enc, uint32_t cols, uint
 
 // The known rows only, packed by rank: T[c * k + i] = enc[rows[i] * cols + c]
 // * Z(w^rows[i]) for the k known rows rows[0..k) (a 32 x 32 tile of (rank,
-// column) per block: whole 128-byte segments both ways).  The block's row
+// column) per block: whole 128-byte segments both ways); a known even row 2j
+// is also the data row j, written as read.  The block's row
 // numbers and factors are read once into LDS, so every value costs one global
 // load, and each thread moves 32 / GATHER_NY values (at 2048^2: 1.73 ms with
 // per-value row and factor loads and 4 values per thread, 1.09 ms so).
 __global__ void dec_gather_ranked(const uint32_t* __restrict__ enc, uint32_t cols, uint32_t k,
                                   const uint32_t* __restrict__ rows, const uint32_t* __restrict__ zk_m,
-                                  uint32_t* __restrict__ T) {
+                                  uint32_t* __restrict__ T, uint32_t* __restrict__ data) {
   __shared__ uint32_t tile[TT][TT + 1];
   __shared__ uint32_t rid[TT], zr[TT];  // the block's 32 row numbers and their Z(w^r)
   const uint32_t c0 = blockIdx.x * TT, i0 = blockIdx.y * TT;
@@ -132,7 +158,11 @@ __global__ void dec_gather_ranked(const uint32_t* __restrict__ enc, uint32_t col
   for (uint32_t q = ty; q < TT; q += GATHER_NY) {
     const uint32_t i = i0 + q, c = c0 + tx;
     uint32_t v = 0;
-    if (i < k && c < cols) v = mont_mul(enc[(size_t)rid[q] * cols + c], zr[q]);
+    if (i < k && c < cols) {
+      const uint32_t r = rid[q], e = enc[(size_t)r * cols + c];
+      v = mont_mul(e, zr[q]);
+      if (!(r & 1u)) data[(size_t)(r >> 1) * cols + c] = e;  // a known data row
+    }
     tile[q][tx] = v;
   }
   __syncthreads();
@@ -202,21 +232,20 @@ __global__ __launch_bounds__(DEC_NT) void dec_fused(uint32_t* __restrict__ T, ui
   for (uint32_t j = threadIdx.x; j < k; j += DEC_NT) col[j] = v[brev_n(j, log_n - 1)];
 }
 
-__constant__ DecodeConst kDecodeConst = make_decode_const(DEC_COSET);
+__constant__ DecodeConst kDecodeConst = make_decode_const();
 
 // Persistent form for 2^8 <= n <= 2^14: tiles of W columns of the
-// rank-packed scratch (k values each) through both FP64 transform pairs in
-// LDS (decode_core.hpp); the k data values of each column replace its entries.
+// rank-packed scratch (k values each) through the FP64 transform pair in LDS
+// (decode_core.hpp); the k even-row values of each column (the data on the
+// erased rows, 0 on the known ones) replace its entries.
 template <class PL>
 __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_columns(uint32_t* __restrict__ T, uint32_t cols, uint32_t ntiles,
                                                       const uint32_t* __restrict__ rows,
-                                                      const uint32_t* __restrict__ zinv, DecodeTabs tabs) {
+                                                      const uint32_t* __restrict__ srow, DecodeTabs tabs) {
   extern __shared__ int32_t lds[];
   dec_load_tables<PL>((int)threadIdx.x, lds, tabs);
-  DecodeTabs ct = tabs;  // the uniform tables from constant memory
+  DecodeTabs ct = tabs;  // the roots from constant memory
   ct.roots = kDecodeConst.roots;
-  ct.mid[0] = kDecodeConst.mid[PL::L - DEC_FP64_MIN_LOG][0];
-  ct.mid[1] = kDecodeConst.mid[PL::L - DEC_FP64_MIN_LOG][1];
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     // A thread's addresses are the same for every tile.  Making its index
     // depend on the tile (through an asm the optimizer cannot see through,
@@ -227,25 +256,15 @@ __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_columns(uint32_t* __rest
     asm("" : "+v"(tid) : "s"(tile));
     dec_load<PL>(tid, tile, cols, T, rows, lds);
     __syncthreads();  // (on the first tile also the tables)
-    dec_step<PL, 0>(tid, lds, ct, zinv);
+    dec_step<PL, 0>(tid, lds, ct, srow);
     __syncthreads();
-    dec_step<PL, 1>(tid, lds, ct, zinv);
+    dec_step<PL, 1>(tid, lds, ct, srow);
     __syncthreads();
-    dec_step<PL, 2>(tid, lds, ct, zinv);
+    dec_step<PL, 2>(tid, lds, ct, srow);
     __syncthreads();
-    dec_step<PL, 3>(tid, lds, ct, zinv);
+    dec_step<PL, 3>(tid, lds, ct, srow);
     __syncthreads();
-    dec_step<PL, 4>(tid, lds, ct, zinv);
-    __syncthreads();
-    dec_step<PL, 5>(tid, lds, ct, zinv);
-    __syncthreads();
-    dec_step<PL, 6>(tid, lds, ct, zinv);
-    __syncthreads();
-    dec_step<PL, 7>(tid, lds, ct, zinv);
-    __syncthreads();
-    dec_step<PL, 8>(tid, lds, ct, zinv);
-    __syncthreads();
-    dec_step<PL, 9>(tid, lds, ct, zinv);
+    dec_step<PL, 4>(tid, lds, ct, srow);
     __syncthreads();
     dec_store<PL>(tid, tile, cols, T, lds);
     __syncthreads();  // the next tile's load overwrites LDS
@@ -259,7 +278,7 @@ int get_decode_tables(DeviceState* st, const DecodeTabs** out) {
   std::lock_guard<std::recursive_mutex> lock(st->mu);
   auto it = st->decode.find((uint32_t)L);
   if (it == st->decode.end()) {
-    const DecodeTableImage im = build_decode_tables<PL>(DEC_COSET);
+    const DecodeTableImage im = build_decode_tables<PL>();
     if (im.packed.empty()) return fail(ZODA_ESTATE, "decoder table layout mismatch for n = 2^%d", L);
     DecodeTableSet set;
     if (hipMalloc((void**)&set.packed, im.packed.size() * sizeof(uint32_t)) != hipSuccess) {
@@ -272,10 +291,8 @@ int get_decode_tables(DeviceState* st, const DecodeTabs** out) {
       (void)hipFree(set.packed);
       return fail((int)e, "copying decoder tables failed: %s", hipGetErrorString(e));
     }
-    set.tabs = new DecodeTabs(make_decode_tabs(L, im, set.packed, DecodeConst{}));
-    // The kernel points these at its constant-memory copy (kDecodeConst).
-    set.tabs->roots = nullptr;
-    set.tabs->mid[0] = set.tabs->mid[1] = nullptr;
+    set.tabs = new DecodeTabs(make_decode_tabs(im, set.packed, DecodeConst{}));
+    set.tabs->roots = nullptr;  // the kernel reads its constant-memory copy (kDecodeConst)
     it = st->decode.emplace((uint32_t)L, set).first;
   }
   *out = it->second.tabs;
@@ -301,20 +318,24 @@ int decode_grid(DeviceState* st, uint32_t* grid) {
   return 0;
 }
 
-// data[j * cols + c] = T[c * stride + j] for j < k.
+// data[j * cols + c] = T[c * stride + j] for j < k, except the rows j whose
+// row 2j is known (`known` non-null: the FP64 path's gather wrote them).
 __global__ void dec_scatter(const uint32_t* __restrict__ T, uint32_t stride, uint32_t k, uint32_t cols,
-                            uint32_t* __restrict__ data) {
+                            const uint8_t* __restrict__ known, uint32_t* __restrict__ data) {
   __shared__ uint32_t tile[TT][TT + 1];
+  __shared__ uint8_t skip[TT];  // data row j0 + i already written
   const uint32_t j0 = blockIdx.x * TT, c0 = blockIdx.y * TT;
   const uint32_t tx = threadIdx.x, ty = threadIdx.y;
+  if (ty == 0) skip[tx] = known && j0 + tx < k ? known[2 * (j0 + tx)] : 0u;
+  __syncthreads();
   for (uint32_t i = ty; i < TT; i += SCATTER_NY) {
     const uint32_t c = c0 + i, j = j0 + tx;
-    tile[i][tx] = (c < cols && j < k) ? T[(size_t)c * stride + j] : 0u;
+    tile[i][tx] = (c < cols && j < k && !skip[tx]) ? T[(size_t)c * stride + j] : 0u;
   }
   __syncthreads();
   for (uint32_t i = ty; i < TT; i += SCATTER_NY) {
     const uint32_t j = j0 + i, c = c0 + tx;
-    if (j < k && c < cols) data[(size_t)j * cols + c] = tile[tx][i];
+    if (j < k && c < cols && !skip[i]) data[(size_t)j * cols + c] = tile[tx][i];
   }
 }
 
@@ -420,10 +441,10 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
     rc = fail(ZODA_ESTATE, "raising the decoder's LDS limit failed");
   if (!rc) {
     hipLaunchKernelGGL(dec_prep, dim3((2 * n + 255) / 256), dim3(256), 0, s, d_x, d_known, d_ep, ne, log_n,
-                       to_mont_host(DEC_COSET), fp64 ? 1 : 0, d_zk, d_zinv);
+                       to_mont_host(DEC_COSET), fp64 ? 1 : 0, to_mont_host(n), d_zk, d_zinv);
     if (fp64) {
       hipLaunchKernelGGL(dec_gather_ranked, dim3((cols + TT - 1) / TT, (k + TT - 1) / TT), dim3(TT, GATHER_NY), 0, s, d_enc,
-                         cols, k, d_rows, d_zk, T);
+                         cols, k, d_rows, d_zk, T, d_data);
       (void)with_decode_plan((int)log_n, [&](auto plan) -> int {
         using PL = decltype(plan);
         const uint32_t ntiles = (cols + PL::W - 1) / PL::W;
@@ -438,7 +459,7 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
                          d_zinv, d_hinv);
     }
     hipLaunchKernelGGL(dec_scatter, dim3((k + TT - 1) / TT, (cols + TT - 1) / TT), dim3(TT, SCATTER_NY), 0, s, T, t_rows, k,
-                       cols, d_data);
+                       cols, fp64 ? d_known : nullptr, d_data);
     if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "decode launch failed");
   }
   if (T && hipFreeAsync(T, s) != hipSuccess && !rc) rc = fail(ZODA_ESTATE, "hipFreeAsync failed");

@@ -2,28 +2,32 @@
 // gfx950 kernel (decode.hip) and a CPU emulation used by the tests.
 //
 // A work-group holds W columns of n = 2^L values (y_r = out[r] * Z(w^r) on
-// the known rows, 0 on the erased ones) in LDS as int32.  Both halves of the decode are the transform pair
+// the known rows, 0 on the erased ones: the values of g = f Z at every w^r)
+// in LDS as int32, and runs one transform pair on them:
 //
-//     NTT_n( t (.) INTT'_n(x) ),   t_f = c * g^f * [f < lim]
+//     v = NTT_n( f (.) INTT'_n(y) ),   (f (.) G)_f = f * G_f
 //
-// (INTT' without its 1/n, folded into c):
-//   first pair:  t_f = h^f / n                    -> g(h w^r) at row r
-//   scaling:     * 1/Z(h w^r)                     -> f(h w^r)
-//   second pair: t_f = h^-f / n for f < n/2, 0    -> f(w^r); the even rows are the data
+// (INTT' without its 1/n): the coefficients of x g'(x) times n, evaluated, so
+// v_r = n w^r g'(w^r).  Where row e is erased, Z(w^e) = 0 and
+// g'(w^e) = f(w^e) Z'(w^e), so the data value there is v_e * s_e with
+// s_e = 1 / (n w^e Z'(w^e)) (a per-row table); the known rows need no
+// arithmetic.  Only the even rows (the data) leave the pair.
 //
-// Each pair runs as five passes over three digits of the row index -- R1
+// The pair runs as five passes over three digits of the row index -- R1
 // (top), R2 and RD (bottom) bits -- in the arrangement of the fused encode
 // (encode_core.hpp): INTT passes on R1 and R2, a middle pass on RD (inverse
-// DFT, twist, forward DFT), NTT passes on R2 and R1.  The INTT passes leave
-// their outputs in digit-reversed order, so the frequency index of the value
-// at a middle-pass position is f = k1 + 2^R1 k2 + 2^(R1+R2) m.  The twist
-// factor c g^(k1 + 2^R1 k2) is constant over a middle-pass group and is
-// multiplied in by the second INTT pass, next to its twiddle; g^(2^(R1+R2) m)
-// [f < lim] comes from a table of 2^RD entries (lim = n/2 is m < 2^(RD-1)).
+// DFT, the multiplication by f, forward DFT), NTT passes on R2 and R1.  The
+// INTT passes leave their outputs in digit-reversed order, so the frequency
+// index of the value at a middle-pass position is f = k1 + 2^R1 k2 +
+// 2^(R1+R2) m.  The middle pass's forward DFT output index is the row's low
+// RD bits; its even outputs are a DFT of half the size of the folded input
+// (z_m + z_(m+H)), and from there on only the groups of even rows are
+// transformed.
 //
 // Bounds: LDS holds |v| < 2^31 (canonical inputs, then values reduced below
-// 2^30 by the constant multiplication that ends every pass); a register DFT
-// of at most 32 points keeps them below 2^36, inside mulmod's 2^45.
+// 2^30 by the constant multiplication or reduction that ends every pass); a
+// register DFT of at most 32 points keeps them below 2^36, inside mulmod's
+// 2^45, and the product by f < 2^14 below 2^50, inside reduce's 2^51.
 #pragma once
 
 #include <stdint.h>
@@ -36,12 +40,10 @@
 namespace zoda {
 
 struct DecodeTabs {
-  // canonical u32, copied into LDS: inv1 | fwd1 | fwd2 | inv2 | fold[0] | fold[1]
+  // canonical u32, copied into LDS: inv1 | fwd1 | fwd2 | inv2
   const uint32_t* packed;
-  uint32_t o_inv1, o_fwd1, o_fwd2, o_inv2, o_fold[2], packed_words;
-  // uniform tables (DecodeConst below; constant memory on the GPU)
-  const Tw* mid[2];  // per-value twist factors of each pair's middle pass
-  const Tw* roots;   // w_64^e for e < 64, then w_64^-e
+  uint32_t o_inv1, o_fwd1, o_fwd2, o_inv2, packed_words;
+  const Tw* roots;  // w_64^e for e < 64, then w_64^-e (constant memory on the GPU)
 };
 
 template <int L_, int W_, int NT_, int R1_, int R2_, int RD_, int MINW_ = 4>
@@ -62,8 +64,8 @@ struct DecodePlan {
   static constexpr bool FT = L >= 13;
   static constexpr int FA = FT ? (LO1 + 1) / 2 : LO1;
   static constexpr int TOP_WORDS = FT ? (1 << (R1 + FA)) + (1 << (R1 + LO1 - FA)) : S;
-  // inv1, fwd1 (TOP_WORDS each), fwd2, inv2 (2^(LO2+R2) each), fold[2] (2^(R1+R2) each)
-  static constexpr int TW_WORDS = 2 * TOP_WORDS + 2 * (1 << (LO2 + R2)) + 2 * (1 << (R1 + R2));
+  // inv1, fwd1 (TOP_WORDS each), fwd2, inv2 (2^(LO2+R2) each)
+  static constexpr int TW_WORDS = 2 * TOP_WORDS + 2 * (1 << (LO2 + R2));
   static constexpr int LDS_WORDS = TILE_WORDS + TW_WORDS;
 };
 
@@ -87,17 +89,16 @@ ZHD double top_twiddle(double x, const uint32_t* __restrict__ t, uint32_t k, uin
 }
 
 // INTT pass on the digit of R bits at bit LO: inverse DFT over the digit,
-// then the inter-digit twiddle from the LDS table `tl` ([k << LO | low]:
-// neighbouring lanes read neighbouring words) and, in the second pass
-// (FOLDED), the twist factor of the middle group from the LDS table `tf`
-// ([high << R | k]).
-template <class PL, int LO, int R, bool FOLDED>
-ZHD void dec_intt(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ tl,
-                  const uint32_t* __restrict__ tf, const Tw* __restrict__ roots) {
+// then the inter-digit twiddle: the top digit's from its table(s) `tl` (see
+// DecodePlan::FT), the second digit's from the LDS table `tl` ([k << LO |
+// low]: neighbouring lanes read neighbouring words).
+template <class PL, int LO, int R>
+ZHD void dec_intt(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ tl, const Tw* __restrict__ roots) {
   constexpr int RR = 1 << R;
+  constexpr bool TOP = LO + R == PL::L;
   constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
   static_assert((PL::W << (PL::L - R)) % PL::NT == 0, "groups must divide evenly over the threads");
-  const TwTab own{nullptr, tl}, fold{nullptr, tf};
+  const TwTab own{nullptr, tl};
 #pragma unroll
   for (int gi = 0; gi < G; gi++) {
     group_fence();
@@ -107,55 +108,52 @@ ZHD void dec_intt(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<LO, R>(u, m), c)];
     dft_reg<R>(x, roots + 64);
-    const uint32_t low = u & ((1u << LO) - 1u), high = (u >> LO) << R;
-    if constexpr (FOLDED) {
+    const uint32_t low = u & ((1u << LO) - 1u);
+    x[0] = reduce(x[0]);  // twiddle w^0 = 1
 #pragma unroll
-      for (int k = 0; k < RR; k++)
-        x[k] = mulmod(mulmod(x[k], tw_get<true>(own, ((uint32_t)k << LO) | low)), tw_get<true>(fold, high + k));
-    } else {
-      static_assert(FOLDED || LO + R == PL::L, "the unfolded pass is the top digit's");
-      x[0] = reduce(x[0]);  // twiddle w^0 = 1
-#pragma unroll
-      for (int k = 1; k < RR; k++) x[k] = top_twiddle<PL>(x[k], tl, (uint32_t)k, low);
+    for (int k = 1; k < RR; k++) {
+      if constexpr (TOP)
+        x[k] = top_twiddle<PL>(x[k], tl, (uint32_t)k, low);
+      else
+        x[k] = mulmod(x[k], tw_get<true>(own, ((uint32_t)k << LO) | low));
     }
 #pragma unroll
     for (int m = 0; m < RR; m++) lds[lds_at<PL>(row_of<LO, R>(u, m), c)] = (int32_t)x[m];
   }
 }
 
-// Middle pass on the bottom digit: inverse DFT, the per-value twist factor,
-// forward DFT, forward twiddle of the digit above (LDS table `ta`).  The
-// forward DFT's output index is the row's low RD bits.  Second pair (DATA):
-// the twist is 0 for m >= 2^(RD-1) and only even rows are kept, and the even
-// outputs of a DFT whose upper half is zero are the DFT of half the size of
-// the lower half, X[2j] = sum_{m < H} z_m (w^2)^(jm); so the pass runs that
-// and writes the even rows only.
-template <class PL, bool DATA>
-ZHD void dec_mid(int tid, int32_t* __restrict__ lds, const Tw* __restrict__ mid, const uint32_t* __restrict__ ta,
-                 const Tw* __restrict__ roots) {
-  constexpr int R = PL::RD, RR = 1 << R, LOA = PL::LO2, RA = PL::R2;
-  constexpr int RO = DATA ? R - 1 : R, RO_N = 1 << RO;  // forward DFT size
+// Middle pass on the bottom digit: inverse DFT, the multiplication of every
+// value by its frequency index f = k1 + 2^R1 k2 + 2^(R1+R2) m (exact below
+// 2^50, then reduced), and the forward DFT's even outputs -- those of a DFT of
+// half the size of the folded values z_m + z_(m+H), X[2q] = sum_{m < H}
+// (z_m + z_(m+H)) (w^2)^(qm) -- each times the forward twiddle of the digit
+// above (LDS table `ta`), written to the even rows.
+template <class PL>
+ZHD void dec_mid(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ ta, const Tw* __restrict__ roots) {
+  constexpr int R = PL::RD, RR = 1 << R, H = RR / 2, LOA = PL::LO2, RA = PL::R2;
   constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
   static_assert((PL::W << (PL::L - R)) % PL::NT == 0, "groups must divide evenly over the threads");
+  constexpr double FSTEP = (double)(1u << (PL::R1 + PL::R2));  // f step per m
   const TwTab above{nullptr, ta};
 #pragma unroll
   for (int gi = 0; gi < G; gi++) {
     group_fence();
     const uint32_t g = (uint32_t)(gi * PL::NT + tid);
-    const uint32_t c = g % PL::W, u = g / PL::W;
+    const uint32_t c = g % PL::W, u = g / PL::W;  // u = k1 << R2 | k2
     double x[RR];
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<0, R>(u, m), c)];
     dft_reg<R>(x, roots + 64);
-    double y[RO_N];
-    y[0] = x[0];  // mid[0] = 1
+    const double f0 = (double)((u >> PL::R2) + ((u & ((1u << PL::R2) - 1u)) << PL::R1));
 #pragma unroll
-    for (int m = 1; m < RO_N; m++) y[m] = mulmod(x[m], mid[m]);
-    dft_reg<RO>(y, roots);
+    for (int m = 0; m < RR; m++) x[m] = reduce(x[m] * (f0 + m * FSTEP));
+    double y[H];
 #pragma unroll
-    for (int q = 0; q < RO_N; q++) {
-      const int m = DATA ? 2 * q : q;  // the row's low RD bits
-      const uint32_t j = row_of<0, R>(u, (uint32_t)m);
+    for (int q = 0; q < H; q++) y[q] = x[q] + x[q + H];
+    dft_reg<R - 1>(y, roots);
+#pragma unroll
+    for (int q = 0; q < H; q++) {
+      const uint32_t j = row_of<0, R>(u, 2u * (uint32_t)q);
       const uint32_t k = (j >> LOA) & ((1u << RA) - 1u);
       y[q] = mulmod(y[q], tw_get<true>(above, (k << LOA) | (j & ((1u << LOA) - 1u))));
       lds[lds_at<PL>(j, c)] = (int32_t)y[q];
@@ -163,13 +161,12 @@ ZHD void dec_mid(int tid, int32_t* __restrict__ lds, const Tw* __restrict__ mid,
   }
 }
 
-// NTT pass on the second digit: forward DFT, forward twiddle of the top digit.
-// Second pair (DATA): only the groups of even rows (the row's low bit is the
-// group's).
-template <class PL, bool DATA>
+// NTT pass on the second digit: forward DFT, forward twiddle of the top
+// digit; only the groups of even rows (the row's low bit is the group's).
+template <class PL>
 ZHD void dec_ntt2(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ ta, const Tw* __restrict__ roots) {
   constexpr int R = PL::R2, RR = 1 << R, LO = PL::LO2, LOA = PL::LO1, RA = PL::R1;
-  constexpr int NG = (PL::W << (PL::L - R)) >> (DATA ? 1 : 0);
+  constexpr int NG = (PL::W << (PL::L - R)) >> 1;
   static_assert(NG % PL::NT == 0 || PL::NT % NG == 0, "groups must divide evenly over the threads");
   constexpr int G = NG >= PL::NT ? NG / PL::NT : 1;
 #pragma unroll
@@ -177,7 +174,7 @@ ZHD void dec_ntt2(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
     group_fence();
     const uint32_t g = (uint32_t)(gi * PL::NT + tid);
     if (NG < PL::NT && g >= (uint32_t)NG) break;  // fewer groups than threads
-    const uint32_t c = g % PL::W, u = (g / PL::W) << (DATA ? 1 : 0);
+    const uint32_t c = g % PL::W, u = (g / PL::W) << 1;
     double x[RR];
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<LO, R>(u, m), c)];
@@ -192,23 +189,22 @@ ZHD void dec_ntt2(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
   }
 }
 
-// Last NTT pass (top digit), natural order out.  First pair (!DATA): every
-// row r times zinv[r] = 1/Z(h w^r) (canonical u32).  Second pair (DATA): only
-// the groups of even rows (the row's low bit is the group's), reduced to
-// canonical u32.
-template <class PL, bool DATA>
-ZHD void dec_ntt1(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ zinv, const Tw* __restrict__ roots) {
+// Last NTT pass (top digit), natural order out, only the groups of even rows:
+// every value times the row's factor srow[r] (1 / (n w^r Z'(w^r)) on erased
+// rows, 0 on known ones, canonical u32), reduced to canonical u32.
+template <class PL>
+ZHD void dec_ntt1(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ srow, const Tw* __restrict__ roots) {
   constexpr int R = PL::R1, RR = 1 << R, LO = PL::LO1;
-  constexpr int NG = (PL::W << (PL::L - R)) >> (DATA ? 1 : 0);
+  constexpr int NG = (PL::W << (PL::L - R)) >> 1;
   static_assert(NG % PL::NT == 0 || PL::NT % NG == 0, "groups must divide evenly over the threads");
   constexpr int G = NG >= PL::NT ? NG / PL::NT : 1;
-  const TwTab zt{nullptr, zinv};
+  const TwTab st{nullptr, srow};
 #pragma unroll
   for (int gi = 0; gi < G; gi++) {
     group_fence();
     const uint32_t g = (uint32_t)(gi * PL::NT + tid);
     if (NG < PL::NT && g >= (uint32_t)NG) break;  // fewer groups than threads
-    const uint32_t c = g % PL::W, u = (g / PL::W) << (DATA ? 1 : 0);
+    const uint32_t c = g % PL::W, u = (g / PL::W) << 1;
     double x[RR];
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<LO, R>(u, m), c)];
@@ -216,10 +212,7 @@ ZHD void dec_ntt1(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
 #pragma unroll
     for (int m = 0; m < RR; m++) {
       const uint32_t row = row_of<LO, R>(u, m);
-      if constexpr (DATA)
-        lds[lds_at<PL>(row, c)] = (int32_t)canonical(x[m]);
-      else
-        lds[lds_at<PL>(row, c)] = (int32_t)mulmod(x[m], tw_get<true>(zt, row));
+      lds[lds_at<PL>(row, c)] = (int32_t)canonical(mulmod(x[m], tw_get<true>(st, row)));
     }
   }
 }
@@ -262,38 +255,29 @@ ZHD void dec_load_tables(int tid, int32_t* __restrict__ lds, const DecodeTabs& T
   for (uint32_t i = (uint32_t)tid; i < T.packed_words; i += PL::NT) dst[i] = T.packed[i];
 }
 
-// Step STEP (0 .. 9) of both transform pairs on the tile in LDS; steps are
+// Step STEP (0 .. 4) of the transform pair on the tile in LDS; steps are
 // separated by a work-group barrier on the GPU.
 template <class PL, int STEP>
-ZHD void dec_step(int tid, int32_t* __restrict__ lds, const DecodeTabs& T, const uint32_t* __restrict__ zinv) {
+ZHD void dec_step(int tid, int32_t* __restrict__ lds, const DecodeTabs& T, const uint32_t* __restrict__ srow) {
   const uint32_t* sb = reinterpret_cast<const uint32_t*>(lds + PL::TILE_WORDS);
-  constexpr int P = STEP / 5, K = STEP % 5;
-  if constexpr (K == 0)
-    dec_intt<PL, PL::LO1, PL::R1, false>(tid, lds, sb + T.o_inv1, nullptr, T.roots);
-  else if constexpr (K == 1)
-    dec_intt<PL, PL::LO2, PL::R2, true>(tid, lds, sb + T.o_inv2, sb + T.o_fold[P], T.roots);
-  else if constexpr (K == 2)
-    dec_mid<PL, P == 1>(tid, lds, T.mid[P], sb + T.o_fwd2, T.roots);
-  else if constexpr (K == 3)
-    dec_ntt2<PL, P == 1>(tid, lds, sb + T.o_fwd1, T.roots);
+  if constexpr (STEP == 0)
+    dec_intt<PL, PL::LO1, PL::R1>(tid, lds, sb + T.o_inv1, T.roots);
+  else if constexpr (STEP == 1)
+    dec_intt<PL, PL::LO2, PL::R2>(tid, lds, sb + T.o_inv2, T.roots);
+  else if constexpr (STEP == 2)
+    dec_mid<PL>(tid, lds, sb + T.o_fwd2, T.roots);
+  else if constexpr (STEP == 3)
+    dec_ntt2<PL>(tid, lds, sb + T.o_fwd1, T.roots);
   else
-    dec_ntt1<PL, P == 1>(tid, lds, zinv, T.roots);
+    dec_ntt1<PL>(tid, lds, srow, T.roots);
 }
 
 // ---- host construction of the tables ------------------------------------------
 
-// Twist factor of the second INTT pass of a pair (digit of r bits, hb bits
-// of the top digit above it): entry [high << r | k] = c * g^(high + 2^hb k).
-inline void push_fold_table(std::vector<Tw>& v, int hb, int r, uint32_t g, uint32_t c) {
-  for (uint32_t high = 0; high < (1u << hb); high++)
-    for (uint32_t k = 0; k < (1u << r); k++)
-      v.push_back(make_tw(mul_mod_host(c, pow_mod_host(g, high + ((uint64_t)k << hb)))));
-}
-
 // Host image of the decoder's LDS tables for n = 2^L (canonical u32).
 struct DecodeTableImage {
   std::vector<uint32_t> packed;
-  uint32_t o_inv1 = 0, o_fwd1 = 0, o_fwd2 = 0, o_inv2 = 0, o_fold[2] = {0, 0};
+  uint32_t o_inv1 = 0, o_fwd1 = 0, o_fwd2 = 0, o_inv2 = 0;
 };
 
 // Inverse twiddles of the digit of r bits at bit lo in [k << lo | low]
@@ -329,9 +313,9 @@ inline void push_top_digit(std::vector<Tw>& v, int lo, int r, bool inverse, bool
 }
 
 template <class PL>
-DecodeTableImage build_decode_tables(uint32_t h) {
+DecodeTableImage build_decode_tables() {
   DecodeTableImage im;
-  constexpr int L = PL::L, r1 = PL::R1, r2 = PL::R2, rd = PL::RD;
+  constexpr int r1 = PL::R1, r2 = PL::R2, rd = PL::RD;
   const int lo1 = rd + r2, lo2 = rd;
   std::vector<Tw> t;
   push_top_digit(t, lo1, r1, true, PL::FT, PL::FA);
@@ -342,38 +326,18 @@ DecodeTableImage build_decode_tables(uint32_t h) {
   push_digit_table(t, lo2, r2, false);
   im.o_inv2 = (uint32_t)t.size();
   push_inverse_digit(t, lo2, r2);
-  const uint32_t n_inv = inv_mod_host(1u << L);
-  const uint32_t g[2] = {h % P, inv_mod_host(h)};
-  for (int p = 0; p < 2; p++) {
-    im.o_fold[p] = (uint32_t)t.size();
-    push_fold_table(t, r1, r2, g[p], n_inv);
-  }
   for (const Tw& x : t) im.packed.push_back((uint32_t)x.w);
   if (im.packed.size() != (size_t)PL::TW_WORDS) im.packed.clear();  // a layout mismatch: no tables
   return im;
 }
 
-// ---- compile-time constant tables (the kernel reads them from constant memory) --
+// ---- compile-time constant table (the kernel reads it from constant memory) --
 
 struct DecodeConst {
-  Tw roots[128];     // w_64^e for e < 64, then w_64^-e
-  Tw mid[7][2][32];  // per plan (L = 8..14) and pair: the middle pass's twist factors
+  Tw roots[128];  // w_64^e for e < 64, then w_64^-e
 };
 
 ZHC Tw tw_const(uint32_t w) { return Tw{(double)w, (double)w / PD}; }
-
-// Middle-pass twist factors of pair p for n = 2^L with digits (r1, r2, rd):
-// g^(2^(r1+r2) m) for m below the pair's limit, 0 above (g = h, then 1/h).
-ZHC void decode_mid(Tw* out, int r1, int r2, int rd, uint32_t h, int p) {
-  const uint32_t g = p == 0 ? h % P : pow_mod_host(h, P - 2);
-  const uint32_t step = pow_mod_host(g, 1ull << (r1 + r2));
-  const uint32_t mlim = p == 0 ? 1u << rd : 1u << (rd - 1);
-  uint32_t cur = 1;
-  for (uint32_t m = 0; m < 32; m++) {
-    out[m] = tw_const(m < (1u << rd) && m < mlim ? cur : 0u);
-    cur = mul_mod_host(cur, step);
-  }
-}
 
 // Decoder plans: n = 2^L for 8 <= L <= 14, W columns so that every pass has
 // at least one group per thread; up to n = 2^12 two 512-thread work-groups
@@ -389,12 +353,7 @@ using DecPlan13 = DecodePlan<13, 1, 256, 5, 4, 4, 2>;
 using DecPlan14 = DecodePlan<14, 1, 512, 5, 5, 4, 2>;
 constexpr int DEC_FP64_MIN_LOG = 8, DEC_FP64_MAX_LOG = 14;
 
-template <class PL>
-ZHC void decode_mid_plan(DecodeConst& c, uint32_t h) {
-  for (int p = 0; p < 2; p++) decode_mid(c.mid[PL::L - DEC_FP64_MIN_LOG][p], PL::R1, PL::R2, PL::RD, h, p);
-}
-
-ZHC DecodeConst make_decode_const(uint32_t h) {
+ZHC DecodeConst make_decode_const() {
   DecodeConst c{};
   const uint32_t w64 = root_of_unity(6), w64i = pow_mod_host(w64, P - 2);
   uint32_t a = 1, b = 1;
@@ -404,18 +363,11 @@ ZHC DecodeConst make_decode_const(uint32_t h) {
     a = mul_mod_host(a, w64);
     b = mul_mod_host(b, w64i);
   }
-  decode_mid_plan<DecPlan8>(c, h);
-  decode_mid_plan<DecPlan9>(c, h);
-  decode_mid_plan<DecPlan10>(c, h);
-  decode_mid_plan<DecPlan11>(c, h);
-  decode_mid_plan<DecPlan12>(c, h);
-  decode_mid_plan<DecPlan13>(c, h);
-  decode_mid_plan<DecPlan14>(c, h);
   return c;
 }
 
-// DecodeTabs over a copy `packed` of the image and the uniform tables `c`.
-inline DecodeTabs make_decode_tabs(int L, const DecodeTableImage& im, const uint32_t* packed, const DecodeConst& c) {
+// DecodeTabs over a copy `packed` of the image and the uniform table `c`.
+inline DecodeTabs make_decode_tabs(const DecodeTableImage& im, const uint32_t* packed, const DecodeConst& c) {
   DecodeTabs T;
   T.packed = packed;
   T.o_inv1 = im.o_inv1;
@@ -423,10 +375,6 @@ inline DecodeTabs make_decode_tabs(int L, const DecodeTableImage& im, const uint
   T.o_fwd2 = im.o_fwd2;
   T.o_inv2 = im.o_inv2;
   T.packed_words = (uint32_t)im.packed.size();
-  for (int p = 0; p < 2; p++) {
-    T.o_fold[p] = im.o_fold[p];
-    T.mid[p] = c.mid[L - DEC_FP64_MIN_LOG][p];
-  }
   T.roots = c.roots;
   return T;
 }
