@@ -7,7 +7,10 @@ tw[h + j] = w_(2h)^j tables, the scalings by h^i / n and h^-i / n -- run on
 small squares encoded by the oracle (src/ntt/cpu.rs transforms, zero-padded
 to 2k as in src/zoda/babybear.rs:195-211) must give the data back for every
 erasure pattern.  A wrong twiddle direction, a missing bit reversal or a
-scaling on the wrong half fails here."""
+scaling on the wrong half fails here.  That is the schedule of the sizes
+without an FP64 plan (dec_fused); the FP64 path's identity (erased rows from
+the formal derivative of g = f Z, one transform pair) is checked the same way
+with the oracle's transforms."""
 import numpy as np
 import pytest
 
@@ -108,3 +111,51 @@ def test_schedule_recovers_data(k):
     for known in patterns:
         for c in range(data.shape[1]):
             assert decode_column(enc[:, c], known, k) == [int(d) for d in data[:, c]], (k, sorted(known), c)
+
+
+def derivative_decode_column(col, known, k):
+    """The FP64 path's identity on one column, with the oracle's transforms
+    (src/ntt/cpu.rs: ntt, intt with its 1/n): g = f Z has the values y, so
+    n w^r g'(w^r) = NTT_n(i * n G_i) and, where Z(w^e) = 0,
+    f(w^e) = w^e g'(w^e) / (w^e Z'(w^e)); known data rows are copied."""
+    n = 2 * k
+    roots = O.roots_of_unity(n)
+    x = [int(roots[r]) for r in range(n)]
+    erased = [r for r in range(n) if r not in known]
+
+    def z_at(pt):
+        acc = 1
+        for e in erased:
+            acc = acc * (pt - x[e]) % P
+        return acc
+
+    y = [int(col[r]) * z_at(x[r]) % P if r in known else 0 for r in range(n)]
+    g = O.intt(np.array(y, dtype=np.uint32), roots, P)        # coefficients of g
+    dg = O.ntt(np.array([i * int(g[i]) % P for i in range(n)], dtype=np.uint32), roots, P)  # w^r g'(w^r)
+    out = []
+    for j in range(k):
+        r = 2 * j
+        if r in known:
+            out.append(int(col[r]))
+            continue
+        zd = 1
+        for e in erased:
+            if e != r:
+                zd = zd * (x[r] - x[e]) % P
+        out.append(int(dg[r]) * pw(x[r] * zd % P, P - 2) % P)
+    return out
+
+
+@pytest.mark.parametrize("k", [1, 2, 4, 16, 64])
+def test_derivative_identity_recovers_data(k):
+    # the decoder's FP64 path: one transform pair and the formal derivative
+    n = 2 * k
+    rng = np.random.default_rng(100 + k)
+    data = rng.integers(0, P, size=(k, 2), dtype=np.uint32)
+    data[0, 0] = P - 1
+    enc = O.encode(data, threads=1)
+    patterns = [set(range(1, n, 2)), set(range(k)), set(range(k, n))]
+    patterns += [set(rng.choice(n, size=k, replace=False).tolist()) for _ in range(3)]
+    for known in patterns:
+        for c in range(data.shape[1]):
+            assert derivative_decode_column(enc[:, c], known, k) == [int(d) for d in data[:, c]], (k, sorted(known), c)
