@@ -163,8 +163,17 @@ extern "C" void zoda_sim_sha256(const uint8_t* data, size_t len, uint8_t* out) {
   h.final(out);
 }
 
+// As the device derives it: roots of whole 64-byte blocks from the shared
+// state after them (one compression per column), others hashed whole.
 extern "C" void zoda_sim_rlc_coefficient(const uint8_t* root, uint32_t len, uint64_t col, uint32_t* out) {
-  rlc_coefficient(root, len, col, out);
+  if (len % 64 == 0) {
+    Sha256 h;
+    h.init();
+    h.update(root, len);
+    rlc_coefficient_mid(h.h, (uint64_t)len * 8 + 64, col, out);
+  } else {
+    rlc_coefficient(root, len, col, out);
+  }
 }
 
 // ---- erasure decoder (decode_core.hpp) ------------------------------------------

@@ -30,7 +30,7 @@ def u32(t):
     return t.cpu().numpy().view(np.uint32)
 
 
-@pytest.mark.parametrize("root", ["", "c0ffee", "a" * 64, "3f" * 40])
+@pytest.mark.parametrize("root", ["", "c0ffee", "a" * 64, "3f" * 40, "e" * 128])
 def test_coefficients_match_oracle(zh, torch, root):
     print(f"[case] rlc coefficients root_len={len(root)}", flush=True)
     cols = 3000

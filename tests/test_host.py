@@ -191,8 +191,10 @@ def test_device_sha256_matches_hashlib():
         assert _sim_sha256(data) == hashlib.sha256(data).digest(), n
 
 
-@pytest.mark.parametrize("root", ["", "ab", "0" * 63, "f" * 64, "9d" * 50])
+@pytest.mark.parametrize("root", ["", "ab", "0" * 63, "f" * 64, "9d" * 50, "e" * 128])
 def test_rlc_coefficient_derivation_matches_oracle(root):
+    # roots of whole 64-byte blocks ("", 64 and 128 characters) take the
+    # shared-state path, the others the whole hash
     import ctypes
     from helpers import sim_lib
     lib = sim_lib()

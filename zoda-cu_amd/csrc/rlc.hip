@@ -39,6 +39,23 @@ __device__ __forceinline__ uint32_t mod_p(uint64_t x) {
   return (uint32_t)(x % 2013265921ull);
 }
 
+// SHA-256 state after a data root of whole 64-byte blocks (the same for every
+// column).
+struct ShaMid {
+  uint32_t h[8];
+};
+
+// Roots of whole blocks: one compression per column from the shared state.
+__global__ void rlc_coeff_mid_kernel(ShaMid mid, uint64_t total_bits, uint32_t cols, uint32_t* __restrict__ coeffs) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  uint32_t limb[4];
+  rlc_coefficient_mid(mid.h, total_bits, c, limb);
+#pragma unroll
+  for (int l = 0; l < 4; l++) coeffs[(size_t)c * 4 + l] = limb[l];
+}
+
+// Other root lengths: the whole hash per column.
 __global__ void rlc_coeff_kernel(const uint8_t* __restrict__ root, uint32_t root_len, uint32_t cols,
                                  uint32_t* __restrict__ coeffs) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -137,6 +154,18 @@ extern "C" int zoda_hip_rlc_coefficients(const char* data_root, uint32_t root_le
   DeviceState* st = nullptr;
   ZODA_TRY(current_state(&st));
   hipStream_t s = (hipStream_t)stream;
+  if (root_len % 64 == 0) {  // e.g. the 64-character hex data root
+    Sha256 h;
+    h.init();
+    h.update(reinterpret_cast<const uint8_t*>(data_root), root_len);
+    ShaMid mid;
+    for (int i = 0; i < 8; i++) mid.h[i] = h.h[i];
+    hipLaunchKernelGGL(rlc_coeff_mid_kernel, dim3((cols + 255) / 256), dim3(256), 0, s, mid,
+                       (uint64_t)root_len * 8 + 64, cols, d_coeffs);
+    if (hipGetLastError() != hipSuccess) return fail(ZODA_ESTATE, "coefficient kernel launch failed");
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(ZODA_ESTATE, "coefficient derivation failed");
+    return 0;
+  }
   uint8_t* d_root = nullptr;
   ZODA_HIP(hipMallocAsync((void**)&d_root, root_len ? root_len : 1, s));
   int rc = 0;

@@ -25,6 +25,17 @@ struct Sha256 {
   }
 
   ZHD void block(const uint8_t* p) {
+    uint32_t w[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+      w[i] = ((uint32_t)p[4 * i] << 24) | ((uint32_t)p[4 * i + 1] << 16) | ((uint32_t)p[4 * i + 2] << 8) |
+             (uint32_t)p[4 * i + 3];
+    compress(w);
+  }
+
+  // One block given as its 16 big-endian message words (overwritten: the
+  // message schedule runs in place).
+  ZHD void compress(uint32_t (&w)[16]) {
     constexpr uint32_t K[64] = {
         0x428a2f98u, 0x71374491u, 0xb5c0fbcfu, 0xe9b5dba5u, 0x3956c25bu, 0x59f111f1u, 0x923f82a4u,
         0xab1c5ed5u, 0xd807aa98u, 0x12835b01u, 0x243185beu, 0x550c7dc3u, 0x72be5d74u, 0x80deb1feu,
@@ -36,11 +47,6 @@ struct Sha256 {
         0x1e376c08u, 0x2748774cu, 0x34b0bcb5u, 0x391c0cb3u, 0x4ed8aa4au, 0x5b9cca4fu, 0x682e6ff3u,
         0x748f82eeu, 0x78a5636fu, 0x84c87814u, 0x8cc70208u, 0x90befffau, 0xa4506cebu, 0xbef9a3f7u,
         0xc67178f2u};
-    uint32_t w[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++)
-      w[i] = ((uint32_t)p[4 * i] << 24) | ((uint32_t)p[4 * i + 1] << 16) | ((uint32_t)p[4 * i + 2] << 8) |
-             (uint32_t)p[4 * i + 3];
     uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
 #pragma unroll
     for (int i = 0; i < 64; i++) {
@@ -112,6 +118,30 @@ struct Sha256 {
   }
 };
 
+// The four limbs from a finished SHA-256 state: limb l is the big-endian u64
+// in digest bytes [8l, 8l + 8) reduced mod p.
+ZHD void rlc_limbs(const uint32_t h[8], uint32_t out[4]) {
+  for (int l = 0; l < 4; l++) out[l] = (uint32_t)((((uint64_t)h[2 * l] << 32) | h[2 * l + 1]) % 2013265921ull);
+}
+
+// RLC coefficient of column `col` for a data root whose length is a multiple
+// of 64 bytes, from `mid`, the SHA-256 state after the root's blocks: the one
+// block left holds the index's 8 little-endian bytes, the 0x80 terminator and
+// the message length in bits (total_bits = root bytes * 8 + 64).
+ZHD void rlc_coefficient_mid(const uint32_t mid[8], uint64_t total_bits, uint64_t col, uint32_t out[4]) {
+  Sha256 s;
+  for (int i = 0; i < 8; i++) s.h[i] = mid[i];
+  const uint32_t lo = (uint32_t)col, hi = (uint32_t)(col >> 32);
+  uint32_t w[16] = {0};
+  w[0] = (lo << 24) | ((lo << 8) & 0xff0000u) | ((lo >> 8) & 0xff00u) | (lo >> 24);  // bytes in order
+  w[1] = (hi << 24) | ((hi << 8) & 0xff0000u) | ((hi >> 8) & 0xff00u) | (hi >> 24);
+  w[2] = 0x80000000u;
+  w[14] = (uint32_t)(total_bits >> 32);
+  w[15] = (uint32_t)total_bits;
+  s.compress(w);
+  rlc_limbs(s.h, out);
+}
+
 // RLC coefficient of column `col`: SHA-256 of the data root's text followed
 // by the column index as 8 little-endian bytes; limb l is the big-endian u64
 // in digest bytes [8l, 8l + 8) reduced mod p (src/zoda/rlc.rs:48-66).
@@ -123,12 +153,8 @@ ZHD void rlc_coefficient(const uint8_t* root, uint32_t root_len, uint64_t col, u
   for (int i = 0; i < 8; i++) le[i] = (uint8_t)(col >> (8 * i));
   s.update(le, 8);
   uint8_t d[32];
-  s.final(d);
-  for (int l = 0; l < 4; l++) {
-    uint64_t v = 0;
-    for (int i = 0; i < 8; i++) v = (v << 8) | d[8 * l + i];
-    out[l] = (uint32_t)(v % 2013265921ull);
-  }
+  s.final(d);  // leaves the digest words in s.h
+  rlc_limbs(s.h, out);
 }
 
 }  // namespace zoda
