@@ -149,6 +149,13 @@ extern "C" int zoda_sim_encode(const uint32_t* in, uint32_t k, uint32_t cols, ui
   return zoda_sim_encode_variant(in, k, cols, n, out, layout, -1);
 }
 
+// The device code's constant multiplication, run on the host: out[i] =
+// mulmod(x[i], w[i], wq) with wq = w/p as the tables hold it (tables != 0) or
+// w * (1/p) as the LDS twiddle reads recompute it.
+extern "C" void zoda_sim_mulmod(const double* x, const double* w, size_t count, int tables, double* out) {
+  for (size_t i = 0; i < count; i++) out[i] = mulmod(x[i], w[i], tables ? w[i] / PD : w[i] * PINV);
+}
+
 // The device code's reduction to canonical form, run on the host, for each of
 // `count` integer-valued doubles.
 extern "C" void zoda_sim_canonical(const double* x, size_t count, uint32_t* out) {

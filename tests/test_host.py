@@ -135,6 +135,34 @@ def test_persistent_kernel_emulation(k, cols):
     assert np.array_equal(sim_encode_persistent(data), O.encode(data, threads=4))
 
 
+@pytest.mark.parametrize("tables", [0, 1])
+def test_constant_multiplication_edges(tables):
+    # the kernels' four-operation multiplication by a constant: exact, congruent
+    # to x * w (mod p) and within p/2 + p/256, for |x| < 2^45 at the ends of the
+    # range, around multiples of p, and at random, against every kind of w
+    import ctypes
+    from helpers import sim_lib
+    lib = sim_lib()
+    rng = np.random.default_rng(7 + tables)
+    xs = [0, 1, -1, P - 1, P, P + 1, -P, 2 ** 45 - 1, -(2 ** 45 - 1), 2 ** 44, -(2 ** 44) + 3,
+          (2 ** 45 // P) * P, -(2 ** 45 // P) * P + 1]
+    xs += rng.integers(-(2 ** 45) + 1, 2 ** 45, size=4000).tolist()
+    ws = [0, 1, 2, P - 1, P - 2, (P - 1) // 2, (P + 1) // 2] + rng.integers(0, P, size=200).tolist()
+    xa = np.array([x for x in xs for _ in ws], dtype=np.float64)
+    wa = np.array([w for _ in xs for w in ws], dtype=np.float64)
+    out = np.zeros_like(xa)
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.zoda_sim_mulmod(xa.ctypes.data_as(dp), wa.ctypes.data_as(dp), ctypes.c_size_t(xa.size), tables,
+                        out.ctypes.data_as(dp))
+    assert np.all(out == np.round(out))
+    assert np.all(np.abs(out) <= P // 2 + P // 256 + 1)
+    xi = xa.astype(np.int64)
+    wi = wa.astype(np.int64)
+    got = out.astype(np.int64) % P
+    expect = ((xi % P) * wi) % P
+    assert np.array_equal(got, expect)
+
+
 def test_canonical_reduction_edges():
     # the kernels' floor-division reduction: every multiple of p in range and
     # its neighbours (j = 0, 1, p - 1 in x = n p + j), and random values over
