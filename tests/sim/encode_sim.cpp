@@ -96,8 +96,7 @@ static int sim_pipelined(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t
   static_assert(PL::D == 3, "the emulated schedule has three digits");
   constexpr int G1 = (PL::W << (PL::L - PL::R2)) / PL::NT;
   constexpr int G2 = (PL::W << (PL::L - PL::RD)) / PL::NT;
-  constexpr int G4 = PL::LOAD_LAST ? (PL::W << (PL::L - PL::R1)) / PL::NT : 0;
-  constexpr int NP = 2 * G1 + G2 + G4;
+  constexpr int NP = 2 * G1 + G2;
   for (uint32_t tile = 0; tile < ntiles; tile++) {
     const bool more = tile + 1 < ntiles;
     for (int tid = 0; tid < PL::NT; tid++)
@@ -106,10 +105,7 @@ static int sim_pipelined(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t
     run_step_loading<PL, BUF, 1>(tile, in, out, cols, lds.data(), T, more, tile + 1, 0, NP, &regs, WORDS);
     run_step_loading<PL, BUF, 2>(tile, in, out, cols, lds.data(), T, more, tile + 1, G1, NP, &regs, WORDS);
     run_step_loading<PL, BUF, 3>(tile, in, out, cols, lds.data(), T, more, tile + 1, G1 + G2, NP, &regs, WORDS);
-    if constexpr (G4 > 0)
-      run_step_loading<PL, BUF, 4>(tile, in, out, cols, lds.data(), T, more, tile + 1, 2 * G1 + G2, NP, &regs, WORDS);
-    else
-      run_late_steps<PL, BUF, 4>(tile, in, out, cols, lds.data(), T);
+    run_late_steps<PL, BUF, 4>(tile, in, out, cols, lds.data(), T);
   }
   return 0;
 }

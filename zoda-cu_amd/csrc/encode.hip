@@ -54,7 +54,7 @@ __global__ __launch_bounds__(PL::NT) void encode_kernel(const uint32_t* __restri
 // thread at once stalls every wave until most of them have returned): the
 // even-row stores go out during the first pass, straight from the registers
 // it reads; the next tile's loads are issued in parts, one per group of the
-// middle passes (and, for plans with LOAD_LAST, of the last pass); the
+// middle passes; the
 // odd-row stores leave the last pass as it produces them.  BUF: every global
 // access goes through a raw buffer with 32-bit offsets from the tile's first
 // column (outputs below 4 GiB).
@@ -68,8 +68,7 @@ __global__ __launch_bounds__(PL::NT) void encode_pipelined(const uint32_t* __res
   constexpr int G1 = (PL::W << (PL::L - (PL::D == 2 ? PL::RD : PL::R2))) / PL::NT;
   constexpr int G2 = PL::D == 3 ? (PL::W << (PL::L - PL::RD)) / PL::NT : 0;
   constexpr int G3 = PL::D == 3 ? G1 : 0;
-  constexpr int G4 = (PL::D == 3 && PL::LOAD_LAST) ? (PL::W << (PL::L - PL::R1)) / PL::NT : 0;
-  constexpr int NP = G1 + G2 + G3 + G4;  // load parts
+  constexpr int NP = G1 + G2 + G3;  // load parts
   extern __shared__ int32_t lds[];
   encode_load_tables<PL>((int)threadIdx.x, lds, T);
   const uint32_t* sb = reinterpret_cast<const uint32_t*>(lds + PL::TILE_WORDS);
@@ -100,13 +99,7 @@ __global__ __launch_bounds__(PL::NT) void encode_pipelined(const uint32_t* __res
         if (more) PP::load_part(tid, ntile, in, cols, raw, G1 + G2 + gi, NP);
       });
       __syncthreads();
-      if constexpr (G4 > 0) {
-        encode_step<PL, 4, true, BUF>(tid, tile, in, out, k, cols, 0, lds, T, [&](int gi) {
-          if (more) PP::load_part(tid, ntile, in, cols, raw, G1 + G2 + G3 + gi, NP);
-        });
-      } else {
-        encode_step<PL, 4, true, BUF>(tid, tile, in, out, k, cols, 0, lds, T);
-      }
+      encode_step<PL, 4, true, BUF>(tid, tile, in, out, k, cols, 0, lds, T);
     } else {
       encode_step<PL, 2, true, BUF>(tid, tile, in, out, k, cols, 0, lds, T);
     }

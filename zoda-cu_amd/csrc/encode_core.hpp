@@ -82,14 +82,12 @@ ZHD Tw tw_get(const TwTab& t, uint32_t i) {
 }
 
 template <int L_, int W_, int NT_, int R1_, int R2_, int RD_, bool TWL_ = false,
-          bool PERSIST_ = false, bool FENCE_ = false, bool LOAD_LAST_ = false>
+          bool PERSIST_ = false, bool FENCE_ = false>
 struct EncodePlan {
   static constexpr int L = L_, W = W_, NT = NT_, R1 = R1_, R2 = R2_, RD = RD_;
   static constexpr bool TWL = TWL_;          // twiddle tables in LDS
   static constexpr bool PERSIST = PERSIST_;  // persistent kernel, first pass pipelined in registers
   static constexpr bool FENCE = FENCE_;      // one group's values live at a time
-  // persistent kernel: the next tile's loads also spread over the last pass
-  static constexpr bool LOAD_LAST = LOAD_LAST_;
   static_assert(R1 + R2 + RD == L, "digits must cover the row index");
   static_assert(R2 == 0 || R1 > 0, "digits fill from the top");
   static_assert(RD <= 5 && R1 <= 5 && R2 <= 5, "register DFTs are at most radix 32");
