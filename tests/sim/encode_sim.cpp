@@ -93,19 +93,24 @@ static int sim_pipelined(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t
   auto raw = [&](int tid) -> Raw& { return *reinterpret_cast<Raw*>(regs.data() + (size_t)tid * WORDS); };
   const uint32_t ntiles = cols / PL::W;
   for (int tid = 0; tid < PL::NT; tid++) PP::load(tid, 0, in, cols, raw(tid));
-  static_assert(PL::D == 3, "the emulated schedule has three digits");
-  constexpr int G1 = (PL::W << (PL::L - PL::R2)) / PL::NT;
-  constexpr int G2 = (PL::W << (PL::L - PL::RD)) / PL::NT;
-  constexpr int NP = 2 * G1 + G2;
+  static_assert(PL::D >= 2, "the pipelined kernel has two or three digits");
+  // load parts over the middle passes, as the kernel counts them
+  constexpr int G1 = (PL::W << (PL::L - (PL::D == 2 ? PL::RD : PL::R2))) / PL::NT;
+  constexpr int G2 = PL::D == 3 ? (PL::W << (PL::L - PL::RD)) / PL::NT : 0;
+  constexpr int NP = PL::D == 3 ? 2 * G1 + G2 : G1;
   for (uint32_t tile = 0; tile < ntiles; tile++) {
     const bool more = tile + 1 < ntiles;
     for (int tid = 0; tid < PL::NT; tid++)
       PP::first(tid, raw(tid), lds.data(), T, inv1,
                 [&](int gi) { PP::store_even(tid, gi, tile, out, cols, raw(tid)[gi]); });
     run_step_loading<PL, BUF, 1>(tile, in, out, cols, lds.data(), T, more, tile + 1, 0, NP, &regs, WORDS);
-    run_step_loading<PL, BUF, 2>(tile, in, out, cols, lds.data(), T, more, tile + 1, G1, NP, &regs, WORDS);
-    run_step_loading<PL, BUF, 3>(tile, in, out, cols, lds.data(), T, more, tile + 1, G1 + G2, NP, &regs, WORDS);
-    run_late_steps<PL, BUF, 4>(tile, in, out, cols, lds.data(), T);
+    if constexpr (PL::D == 3) {
+      run_step_loading<PL, BUF, 2>(tile, in, out, cols, lds.data(), T, more, tile + 1, G1, NP, &regs, WORDS);
+      run_step_loading<PL, BUF, 3>(tile, in, out, cols, lds.data(), T, more, tile + 1, G1 + G2, NP, &regs, WORDS);
+      run_late_steps<PL, BUF, 4>(tile, in, out, cols, lds.data(), T);
+    } else {
+      run_late_steps<PL, BUF, 2>(tile, in, out, cols, lds.data(), T);
+    }
   }
   return 0;
 }

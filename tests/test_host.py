@@ -125,7 +125,7 @@ def test_batched_plans_emulation(k, variant):
     assert np.array_equal(sim_encode(data, variant=variant), O.encode(data, threads=4))
 
 
-@pytest.mark.parametrize("k,cols", [(512, 160), (1024, 96), (2048, 48), (4096, 24)])
+@pytest.mark.parametrize("k,cols", [(256, 320), (512, 160), (1024, 96), (2048, 48), (4096, 24)])
 def test_persistent_kernel_emulation(k, cols):
     # first pass from registers loaded while the previous tile was in flight,
     # in parts over the middle passes; even-row stores during the first pass

@@ -19,7 +19,7 @@ using Plan4 = EncodePlan<4, 256, 256, 0, 0, 4>;
 using Plan5 = EncodePlan<5, 256, 256, 0, 0, 5>;
 using Plan6 = EncodePlan<6, 64, 256, 3, 0, 3, true>;
 using Plan7 = EncodePlan<7, 64, 256, 4, 0, 3, true>;
-using Plan8 = EncodePlan<8, 64, 512, 4, 0, 4, true>;
+using Plan8 = EncodePlan<8, 64, 512, 4, 0, 4, true, true>;
 using Plan9 = EncodePlan<9, 32, 512, 3, 3, 3, true, true>;
 using Plan10 = EncodePlan<10, 32, 1024, 4, 3, 3, true, true>;
 using Plan11 = EncodePlan<11, 16, 1024, 4, 4, 3, true, true>;
