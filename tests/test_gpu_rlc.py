@@ -60,6 +60,24 @@ def test_rlc_rows_match_oracle(zh, torch, rows, cols):
     assert not got[3].any()
 
 
+@pytest.mark.parametrize("rows,cols", [(70, 4196), (3, 2048)])
+def test_rlc_rows_any_u32(zh, torch, rows, cols):
+    # rlc_row is defined for any u32 values and coefficients (each reduced
+    # mod p in the product), not only canonical ones: full-range words,
+    # including 2^32 - 1 and values just above p and 2p.
+    print(f"[case] rlc rows any-u32 {rows}x{cols}", flush=True)
+    rng = np.random.default_rng(rows * 7 + cols)
+    sq = rng.integers(0, 2**32, size=(rows, cols), dtype=np.uint64).astype(np.uint32)
+    sq[0, :4] = [2**32 - 1, P, 2 * P, 2 * P - 1]
+    c = rng.integers(0, 2**32, size=(cols, 4), dtype=np.uint64).astype(np.uint32)
+    c[0] = [2**32 - 1, P, 2 * P, 3 * P - 1 - 2**31]
+    t_sq = torch.from_numpy(sq.view(np.int32)).cuda()
+    coeffs = torch.from_numpy(c.view(np.int32)).cuda()
+    got = u32(zh.rlc_rows(t_sq, coeffs))
+    for r in range(rows):
+        assert np.array_equal(got[r], O.rlc_row(sq[r], c)), r
+
+
 def test_commutes_with_the_extension_2048(zh, torch):
     # RLC(extended row i) = sum_j G[i][j] RLC(original row j) for every one of
     # the 4096 rows of the 2048^2 square: each limb of the original rows' RLC,
