@@ -71,11 +71,16 @@ struct TwTab {
   const uint32_t* s;   // LDS u32 (plans with TWL)
 };
 
+// A u32 twiddle as the (w, w/p) pair the multiplication takes.
+ZHD Tw tw_u32(uint32_t v) {
+  const double w = (double)v;
+  return Tw{w, w * PINV};
+}
+
 template <bool LDS_TW>
 ZHD Tw tw_get(const TwTab& t, uint32_t i) {
   if constexpr (LDS_TW) {
-    const double w = (double)t.s[i];
-    return Tw{w, w * PINV};
+    return tw_u32(t.s[i]);
   } else {
     return t.g[i];
   }
@@ -298,6 +303,24 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
         else
           x[m] = (double)lds[row_of<LO, R>(u, m) * PL::W + c];
       }
+      // The group's u32 twiddles (LDS tables) are read before its DFT, so that
+      // their LDS latency hides behind the DFT's arithmetic.
+      constexpr bool HOIST = PL::TWL && (KIND == PASS_INTT || (KIND == PASS_NTT && !LAST));
+      uint32_t twr[HOIST ? RR : 1];
+      if constexpr (HOIST) {
+        if constexpr (KIND == PASS_INTT) {
+          const uint32_t hb = ((u >> LO) << (LO + R)) | ((u & ((1u << LO) - 1u)) << R);
+#pragma unroll
+          for (int k = 0; k < RR; k++) twr[k] = own.s[hb + k];
+        } else {
+#pragma unroll
+          for (int m = 0; m < RR; m++) {
+            const uint32_t j = row_of<LO, R>(u, m);
+            const uint32_t k = (j >> LOA) & ((1u << RA) - 1u);
+            twr[m] = above.s[(k << LOA) | (j & ((1u << LOA) - 1u))];
+          }
+        }
+      }
       if constexpr (KIND == PASS_INTT) {
         dft_reg<R>(x, rinv);
         const uint32_t base = ((u >> LO) << (LO + R)) | ((u & ((1u << LO) - 1u)) << R);
@@ -305,11 +328,11 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
         if constexpr (LAST_INTT) {
           // the table folds in the twist's per-group factor
 #pragma unroll
-          for (int k = 0; k < RR; k++) x[k] = mulmod(x[k], tw_get<PL::TWL>(own, base + k));
+          for (int k = 0; k < RR; k++) x[k] = HOIST ? mulmod(x[k], tw_u32(twr[k])) : mulmod(x[k], tw_get<PL::TWL>(own, base + k));
         } else {
           x[0] = reduce(x[0]);  // twiddle w^0 = 1
 #pragma unroll
-          for (int k = 1; k < RR; k++) x[k] = mulmod(x[k], tw_get<PL::TWL>(own, base + k));
+          for (int k = 1; k < RR; k++) x[k] = HOIST ? mulmod(x[k], tw_u32(twr[k])) : mulmod(x[k], tw_get<PL::TWL>(own, base + k));
         }
       } else {
         if constexpr (KIND == PASS_MID) {
@@ -335,7 +358,10 @@ ZHD void encode_pass(int tid, uint32_t tile, const uint32_t* __restrict__ in,
             const uint32_t j = row_of<LO, R>(u, m);
             const uint32_t k = (j >> LOA) & ((1u << RA) - 1u);
             constexpr bool U32 = PL::TWL && !(PL::MID_PAIRS && KIND == PASS_MID);
-            x[m] = mulmod(x[m], tw_get<U32>(above, (k << LOA) | (j & ((1u << LOA) - 1u))));
+            if constexpr (HOIST)
+              x[m] = mulmod(x[m], tw_u32(twr[m]));
+            else
+              x[m] = mulmod(x[m], tw_get<U32>(above, (k << LOA) | (j & ((1u << LOA) - 1u))));
           }
         }
       }
