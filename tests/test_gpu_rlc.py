@@ -78,6 +78,24 @@ def test_rlc_rows_any_u32(zh, torch, rows, cols):
         assert np.array_equal(got[r], O.rlc_row(sq[r], c)), r
 
 
+def test_rlc_rows_unaligned_square(zh, torch):
+    # a square starting 4 bytes past a 16-byte boundary, with whole 2048-column
+    # blocks: the kernel must not take its 16-byte load path there
+    rows, cols = 20, 4096
+    print(f"[case] rlc rows unaligned {rows}x{cols}", flush=True)
+    rng = np.random.default_rng(4097)
+    sq = rng.integers(0, P, size=(rows, cols), dtype=np.uint32)
+    flat = torch.empty(rows * cols + 1, dtype=torch.int32, device="cuda")
+    t_sq = flat[1:].view(rows, cols)
+    assert t_sq.data_ptr() % 16 == 4
+    t_sq.copy_(torch.from_numpy(sq.view(np.int32)))
+    coeffs = zh.rlc_coefficients("9" * 64, cols)
+    got = u32(zh.rlc_rows(t_sq, coeffs))
+    c = u32(coeffs)
+    for r in range(rows):
+        assert np.array_equal(got[r], O.rlc_row(sq[r], c)), r
+
+
 def test_commutes_with_the_extension_2048(zh, torch):
     # RLC(extended row i) = sum_j G[i][j] RLC(original row j) for every one of
     # the 4096 rows of the 2048^2 square: each limb of the original rows' RLC,
