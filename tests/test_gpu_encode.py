@@ -129,6 +129,72 @@ def test_host_end_to_end(zh, torch):
     assert np.array_equal(pin_out.array.reshape(2 * side, cols), expect)
 
 
+@pytest.mark.parametrize("side,tiles", [(64, 5), (128, 7), (256, 9), (512, 37), (1024, 13),
+                                        (2048, 29), (4096, 11)])
+def test_whole_tiles_column_major(zh, torch, side, tiles):
+    # cols a multiple of 64 (every plan's tile width divides it) and k = 2^L:
+    # the whole-tile instantiations of the column-major layout, the form
+    # zoda_hip_encode_host(..., ZODA_LAYOUT_COL) takes for its slabs
+    cols = 64 * tiles
+    rng = np.random.default_rng(side + tiles)
+    data = rng.integers(0, P, size=(side, cols), dtype=np.uint32)
+    print_case("whole-tiles-col", side=side, cols=cols)
+    expect = O.encode(data, threads=16)
+    got = gpu_encode(zh, torch, data, layout=1)
+    assert got.shape == (cols, 2 * side)
+    assert np.array_equal(got.T, expect)
+    # and the row layout of the same whole tiles
+    assert np.array_equal(gpu_encode(zh, torch, data), expect)
+
+
+def test_column_major_slab_2048(zh, torch):
+    # the exact slab zoda_hip_encode_host encodes at 2048^2 (4096 columns of
+    # 2048 rows, whole tiles, column-major output), every value checked
+    side, cols = 2048, 4096
+    t = torch.empty((side, cols), dtype=torch.int32, device="cuda")
+    zh.fill_uniform(t, seed=77)
+    print_case("slab-col", side=side, cols=cols)
+    out = zh.encode_tensor(t, layout=1)
+    torch.cuda.synchronize()
+    data = to_host(t)
+    assert np.array_equal(to_host(out).T, O.encode(data, threads=16))
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_host_end_to_end_2048(zh, torch, layout):
+    # zoda_hip_encode_host on the whole 2048^2 square from pinned host memory
+    # (the drop-in for encode_gpu, src/benchmarks/utils.rs:100-146): seeded
+    # sampled columns against the oracle and the systematic rows everywhere
+    side = 2048
+    cols = 128 * side
+    rows_out = 2 * side
+    t = torch.empty((side, cols), dtype=torch.int32, device="cuda")
+    zh.fill_uniform(t, seed=4242 + layout)
+    pin_in = zh.PinnedSquare(side * cols, 1)
+    pin_out = zh.PinnedSquare(rows_out * cols, 1)
+    try:
+        a_in = pin_in.array.reshape(side, cols)
+        a_in[:] = to_host(t)
+        del t
+        torch.cuda.empty_cache()
+        shape = (rows_out, cols) if layout == 0 else (cols, rows_out)
+        a_out = pin_out.array.reshape(shape)
+        print_case("host-2048", layout=layout)
+        zh.encode_host(a_in, layout=layout, out=a_out)
+        rng = np.random.default_rng(side + layout)
+        idx = sorted(set(rng.choice(cols, size=32, replace=False).tolist()) | {0, 4095, 4096, cols - 1})
+        got_s = a_out[:, idx] if layout == 0 else a_out[idx, :].T
+        assert np.array_equal(got_s, O.encode(np.ascontiguousarray(a_in[:, idx]), threads=16))
+        for c0 in range(0, cols, 8192):
+            if layout == 0:
+                assert np.array_equal(a_out[0::2, c0:c0 + 8192], a_in[:, c0:c0 + 8192])
+            else:
+                assert np.array_equal(a_out[c0:c0 + 8192, 0::2], a_in[:, c0:c0 + 8192].T)
+    finally:
+        pin_in.free()
+        pin_out.free()
+
+
 def big_square_checks(zh, torch, side, seed):
     cols = 128 * side
     t_in = torch.empty((side, cols), dtype=torch.int32, device="cuda")

@@ -112,7 +112,10 @@ def test_kernel_emulation_full_tiles(k, cols):
     # whole tiles and k = 2^L select the variant compiled without bounds checks
     rng = np.random.default_rng(k + cols)
     data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)
-    assert np.array_equal(sim_encode(data), O.encode(data, threads=4))
+    expect = O.encode(data, threads=4)
+    assert np.array_equal(sim_encode(data), expect)
+    # the column-major layout of the same whole tiles
+    assert np.array_equal(sim_encode(data, layout=1).T, expect)
 
 
 @pytest.mark.parametrize("variant", [0, 1])
