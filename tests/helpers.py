@@ -98,6 +98,8 @@ def sim_lib():
         lib.zoda_sim_rlc_coefficient.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p]
         lib.zoda_sim_decode.restype = ctypes.c_int
         lib.zoda_sim_decode.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p]
+        lib.zoda_sim_encode_slice.restype = ctypes.c_int
+        lib.zoda_sim_encode_slice.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p]
         lib.zoda_sim_encode_persistent.restype = ctypes.c_int
         lib.zoda_sim_encode_persistent.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p]
         _sim = lib
@@ -111,6 +113,17 @@ def sim_encode_persistent(data: np.ndarray) -> np.ndarray:
     rc = sim_lib().zoda_sim_encode_persistent(d.ctypes.data, k, cols, out.ctypes.data)
     if rc != 0:
         raise ValueError(f"no persistent plan for k={k}, cols={cols} (status {rc})")
+    return out
+
+
+def sim_encode_slice(data: np.ndarray) -> np.ndarray:
+    """The slice kernel (side 2048, 32-column tiles) run on the host."""
+    k, cols = data.shape
+    out = np.zeros((2 * k, cols), dtype=np.uint32)
+    d = np.ascontiguousarray(data, dtype=np.uint32)
+    rc = sim_lib().zoda_sim_encode_slice(d.ctypes.data, k, cols, out.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"no slice plan for k={k}, cols={cols} (status {rc})")
     return out
 
 

@@ -14,6 +14,9 @@
 #include "../../zoda-cu_amd/csrc/sha256.hpp"
 #include "../../zoda-cu_amd/csrc/encode_tables.hpp"
 #include "../../zoda-cu_amd/csrc/decode_core.hpp"
+#include "../../zoda-cu_amd/csrc/slice_core.hpp"
+
+#include <array>
 
 using namespace zoda;
 
@@ -128,6 +131,125 @@ extern "C" int zoda_sim_encode_persistent(const uint32_t* in, uint32_t k, uint32
     }
     else return -4;
   });
+}
+
+// The slice kernel (encode_slice) on one work-group: tile `it` runs its
+// front steps (P1, P2) beside the back steps (P2', P1') of tile it - 1, step
+// by step with the same order of the two phases as the kernel, so the owner
+// registers change hands slot by slot and the LDS staging of the input rows
+// and the odd rows is read and written exactly as on the device; the middle
+// passes run between.  16-byte accesses, zero-record buffers for inactive
+// parts, and the loads one step ahead in per-thread registers, as on the GPU.
+namespace {
+using SP2048 = slice::Plan2048;
+struct SliceSim {
+  using PL = SP2048;
+  using Held = int32_t[PL::RB][PL::RC];
+  const uint32_t* in;
+  uint32_t* out;
+  uint32_t cols;
+  std::vector<slice::Tw2> t1;
+  std::vector<double> t2, t3;
+  std::vector<int32_t> lds;
+  std::vector<std::array<int32_t, PL::RB * PL::RC>> held;
+  std::vector<std::array<U4, PL::QI>> ld;
+  slice::MulK k;
+  Held& h(int tid) { return *reinterpret_cast<Held*>(held[tid].data()); }
+  U4 (&l(int tid))[PL::QI] { return *reinterpret_cast<U4(*)[PL::QI]>(ld[tid].data()); }
+
+  struct Bufs {
+    GBuf ibf, obf, ibn, obn, obb, obp;
+  };
+
+  template <int SP>
+  void step(const Bufs& b, bool front, bool back) {
+    const uint32_t c4 = cols * 4;
+    for (int tid = 0; tid < PL::NT; tid++) {
+      const slice::Tid t(tid);
+      const slice::Offs<PL> o(t, cols);
+      slice::store_odd<PL>(t, SP == 0 ? b.obp : b.obb, o, SP == 0 ? PL::STEPS - 1 : SP - 1, c4, lds.data());
+      if (front) slice::p1<PL>(t, lds.data(), t1.data() + t.w * PL::RA, k);
+      if (back) slice::p2b<PL, SP>(t, h(tid), lds.data(), t3.data(), k);
+    }
+    for (int tid = 0; tid < PL::NT; tid++) {
+      const slice::Tid t(tid);
+      const slice::Offs<PL> o(t, cols);
+      if (front) slice::p2<PL, SP>(t, lds.data(), t2.data(), h(tid), k);
+      if (back) slice::p1b<PL>(t, lds.data(), k);
+      if (SP + 1 < PL::STEPS)
+        slice::stage_step<PL, !PL::EVEN_IN_MID>(t, b.obf, o, SP + 1, c4, l(tid), lds.data());
+      else
+        slice::stage_step<PL, !PL::EVEN_IN_MID>(t, b.obn, o, 0, c4, l(tid), lds.data());
+      if (SP + 2 < PL::STEPS)
+        slice::load_step<PL>(b.ibf, o, t.w, SP + 2, c4, l(tid));
+      else
+        slice::load_step<PL>(b.ibn, o, t.w, SP + 2 - PL::STEPS, c4, l(tid));
+    }
+    if constexpr (SP + 1 < PL::STEPS) step<SP + 1>(b, front, back);
+  }
+
+  // the middle passes; with EVEN_IN_MID one step of even rows is copied per
+  // group, as on the device
+  template <int KB>
+  void mid(int tid, const Bufs& b) {
+    const slice::Tid t(tid);
+    const slice::Offs<PL> o(t, cols);
+    U4 ev[PL::QI];
+    if constexpr (PL::EVEN_IN_MID) slice::load_step<PL>(b.ibf, o, t.w, KB, cols * 4, ev);
+    slice::mid_group<PL, KB>(h(tid), k);
+    if constexpr (PL::EVEN_IN_MID)
+      for (int i = 0; i < PL::QI; i++) b.obf.store4(ev[i], o.ev, slice::Offs<PL>::s_out(KB, t.w, i, cols * 4));
+    if constexpr (KB + 1 < PL::RB) mid<KB + 1>(tid, b);
+  }
+
+  GBuf gb(const uint32_t* base, uint32_t tile, bool live) const { return GBuf(base + (size_t)tile * PL::W, live); }
+
+  void run() {
+    const uint32_t ntiles = cols / PL::W, c4 = cols * 4;
+    for (int tid = 0; tid < PL::NT; tid++) {
+      const slice::Tid t(tid);
+      const slice::Offs<PL> o(t, cols);
+      slice::load_step<PL>(gb(in, 0, true), o, t.w, 0, c4, l(tid));
+      slice::stage_step<PL, !PL::EVEN_IN_MID>(t, gb(out, 0, true), o, 0, c4, l(tid), lds.data());
+      slice::load_step<PL>(gb(in, 0, true), o, t.w, 1, c4, l(tid));
+    }
+    uint32_t tp = 0;
+    for (uint32_t it = 0; it <= ntiles; it++) {
+      const bool front = it < ntiles, back = it > 0, next = it + 1 < ntiles, prev = it > 1;
+      const uint32_t tf = front ? it : 0, tb = back ? it - 1 : 0, tn = next ? it + 1 : 0;
+      const Bufs b{gb(in, tf, front), gb(out, tf, front), gb(in, tn, next), gb(out, tn, next),
+                   gb(out, tb, back), gb(out, tp, prev)};
+      step<0>(b, front, back);
+      if (front)
+        for (int tid = 0; tid < PL::NT; tid++) mid<0>(tid, b);
+      tp = tb;
+    }
+    for (int tid = 0; tid < PL::NT; tid++) {
+      const slice::Tid t(tid);
+      slice::store_odd<PL>(t, gb(out, tp, true), slice::Offs<PL>(t, cols), PL::STEPS - 1, c4, lds.data());
+    }
+  }
+};
+}  // namespace
+
+extern "C" int zoda_sim_encode_slice(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out) {
+  using PL = SP2048;
+  if (k != (uint32_t)PL::S || cols == 0 || cols % PL::W != 0) return -3;
+  if (((uint64_t)(2 * PL::S - 1) * cols + PL::W) * 4 >= (1ull << 32)) return -3;
+  SliceSim sim;
+  sim.in = in;
+  sim.out = out;
+  sim.cols = cols;
+  sim.t1.resize(PL::T1_PAIRS);
+  sim.t2.resize(PL::T2_WORDS);
+  sim.t3.resize(PL::T3_WORDS);
+  slice::build_slice_tables<PL>(sim.t1.data(), sim.t2.data(), sim.t3.data());
+  sim.lds.assign(4 * PL::XWORDS, 0);
+  sim.held.assign(PL::NT, {});
+  sim.ld.assign(PL::NT, {});
+  sim.k = slice::make_mulk();
+  sim.run();
+  return 0;
 }
 
 // variant < 0: the single-square plan of the side; 0/1: the batched plans.

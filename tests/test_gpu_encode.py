@@ -147,6 +147,19 @@ def test_whole_tiles_column_major(zh, torch, side, tiles):
     assert np.array_equal(gpu_encode(zh, torch, data), expect)
 
 
+@pytest.mark.parametrize("tiles", [1, 7, 255, 256, 257, 600])
+def test_slice_kernel_tile_counts(zh, torch, tiles):
+    # side 2048 in the row layout with whole 32-column tiles runs the
+    # slice-pipelined kernel: one persistent work-group per CU, so fewer tiles
+    # than CUs, exactly one per CU, one more, and several per CU; every value
+    # of the encoded square against the oracle
+    side, cols = 2048, 32 * tiles
+    rng = np.random.default_rng(tiles)
+    data = rng.integers(0, P, size=(side, cols), dtype=np.uint32)
+    print_case("slice-tiles", side=side, cols=cols)
+    assert np.array_equal(gpu_encode(zh, torch, data), O.encode(data, threads=16))
+
+
 def test_column_major_slab_2048(zh, torch):
     # the exact slab zoda_hip_encode_host encodes at 2048^2 (4096 columns of
     # 2048 rows, whole tiles, column-major output), every value checked

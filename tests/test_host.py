@@ -118,6 +118,24 @@ def test_kernel_emulation_full_tiles(k, cols):
     assert np.array_equal(sim_encode(data, layout=1).T, expect)
 
 
+@pytest.mark.parametrize("cols", [32, 64, 96])
+def test_slice_kernel_emulation(cols):
+    # the slice-pipelined side-2048 kernel's per-thread code and order of work
+    # (front of tile i beside the back of tile i - 1, held registers handed
+    # over slot by slot, input and odd rows staged in LDS), run on the host
+    from helpers import sim_encode_slice
+    rng = np.random.default_rng(cols)
+    data = rng.integers(0, P, size=(2048, cols), dtype=np.uint32)
+    assert np.array_equal(sim_encode_slice(data), O.encode(data, threads=4))
+
+
+def test_slice_kernel_emulation_extreme_values():
+    from helpers import sim_encode_slice
+    for fill in (0, 1, P - 1):
+        data = np.full((2048, 32), fill, dtype=np.uint32)
+        assert np.array_equal(sim_encode_slice(data), O.encode(data, threads=4))
+
+
 @pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("k", [1, 3, 8, 16, 17, 32, 64, 100, 128, 256, 512, 1024, 2048, 4096])
 def test_batched_plans_emulation(k, variant):

@@ -85,6 +85,8 @@ struct DeviceState {
   // Per-kernel launch set-up done once: persistent grid size (0: the LDS
   // limit is raised, no grid needed).
   std::map<const void*, uint32_t> kernel_setup;
+  // Twiddle tables of the slice-pipelined 2048 encode (encode_slice.hip).
+  void* slice_tables = nullptr;
 };
 
 // State of the calling thread's current device (created on first use).
@@ -112,6 +114,18 @@ int get_encode_tables(DeviceState* st, int L, int r1, int r2, int rd, const Enco
 // the set-up calls are not allowed inside a stream capture.
 int launch_encode(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols,
                   uint32_t n, uint32_t* d_out, int layout, hipStream_t stream, bool dry = false);
+
+// Once per device and kernel: raise the dynamic LDS limit when the kernel
+// needs more than 64 KiB and, for a persistent kernel, size its grid to fill
+// every CU (*grid; 0 otherwise).
+int kernel_setup(DeviceState* st, const void* fn, int nt, size_t lds_bytes, bool persistent,
+                 uint32_t* grid);
+
+// The slice-pipelined encode of side 2048 (whole 32-column tiles, row layout,
+// output below 4 GiB from each tile's base): true when it takes the shape.
+bool slice_fits(uint32_t k, uint32_t cols, int layout);
+int launch_slice(DeviceState* st, const uint32_t* d_in, uint32_t cols, uint32_t* d_out,
+                 hipStream_t stream, bool dry);
 
 // Squares of side >= 2^DIRECT_MIN_LOG fill the device with their own fused
 // kernel: batched and served work runs them as separate launches next to the

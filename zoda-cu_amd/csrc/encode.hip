@@ -7,16 +7,6 @@
 
 namespace zoda {
 
-// Work-groups are handed to the eight XCDs round-robin (block b goes to XCD
-// b % 8).  Give each XCD a contiguous run of tiles, in dispatch order, so that
-// neighbouring tiles -- which share the 128-byte lines of every row -- run at
-// the same time behind the same L2.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
-  const uint32_t x = b & 7u, i = b >> 3;
-  const uint32_t q = nb >> 3, r = nb & 7u;
-  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-}
-
 // LAYOUT is a template parameter so each variant addresses its output with
 // one base and a constant stride per value.
 template <class PL, bool FULL, int LAYOUT>
@@ -140,8 +130,8 @@ int get_encode_tables(DeviceState* st, int L, int r1, int r2, int rd, const Enco
 // Once per device and kernel: raise the dynamic LDS limit when the kernel
 // needs more than the default 64 KiB and, for a persistent kernel, size its
 // grid to fill every CU (*grid; 0 otherwise).
-static int kernel_setup(DeviceState* st, const void* fn, int nt, size_t lds_bytes, bool persistent,
-                        uint32_t* grid) {
+int kernel_setup(DeviceState* st, const void* fn, int nt, size_t lds_bytes, bool persistent,
+                 uint32_t* grid) {
   std::lock_guard<std::recursive_mutex> lock(st->mu);
   auto it = st->kernel_setup.find(fn);
   if (it != st->kernel_setup.end()) {
@@ -173,6 +163,10 @@ static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32
   const size_t lds_bytes = (size_t)PL::LDS_WORDS * sizeof(int32_t);
   const bool full = cols % PL::W == 0 && k == (uint32_t)PL::S;
   const bool row = layout == ZODA_LAYOUT_ROW;
+  // side 2048: the slice-pipelined kernel (32-column tiles) where it fits
+  if constexpr (PL::L == 11) {
+    if (slice_fits(k, cols, layout)) return launch_slice(st, d_in, cols, d_out, stream, dry);
+  }
   if constexpr (PL::PERSIST) {
     // Pipe addresses rows within a digit with 32-bit byte offsets; whole
     // arrays of at most 4 GiB also take the raw-buffer form.

@@ -167,6 +167,16 @@ ZHD uint32_t row_of(uint32_t u, uint32_t m) {
   return (u & ((1u << LO) - 1u)) | (m << LO) | ((u >> LO) << (LO + R));
 }
 
+// Work-groups are handed to the eight XCDs round-robin (block b goes to XCD
+// b % 8).  Give each XCD a contiguous run of tiles, in dispatch order, so that
+// neighbouring tiles -- which share the 128-byte lines of every row -- run at
+// the same time behind the same L2.
+ZHD uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+  const uint32_t x = b & 7u, i = b >> 3;
+  const uint32_t q = nb >> 3, r = nb & 7u;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 ZHD size_t out_index(uint32_t row, uint32_t cidx, uint32_t cols, uint32_t out_rows, int layout) {
   return layout == 0 ? (size_t)row * cols + cidx : (size_t)cidx * out_rows + row;
 }
@@ -193,6 +203,11 @@ struct NoHook {
   ZHD void operator()(int) const {}
 };
 
+// Four adjacent u32 (one 16-byte access).
+struct alignas(16) U4 {
+  uint32_t x, y, z, w;
+};
+
 // Raw-buffer view of a global u32 array.  An access names a per-thread 32-bit
 // byte offset and a uniform 32-bit byte offset, and the buffer instruction adds
 // both to the base itself, so the vector ALU computes no 64-bit address per
@@ -206,6 +221,16 @@ struct GBuf {
   // dword 3 = 0x00020000: 32-bit data format, the gfx9 family's raw buffer
   __device__ explicit GBuf(const uint32_t* p)
       : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(p), 0, -1, 0x00020000)) {}
+  // live = false: zero records, so the hardware drops every store and every
+  // load returns 0 -- memory instructions that need no branch around them
+  // (the record count is pinned to a scalar register: a descriptor in vector
+  // registers turns every access into a loop over its lanes)
+  __device__ GBuf(const uint32_t* p, bool live) : r(make(p, live)) {}
+  static __device__ __amdgpu_buffer_rsrc_t make(const uint32_t* p, bool live) {
+    int nr = __builtin_amdgcn_readfirstlane(live ? -1 : 0);
+    asm volatile("" : "+s"(nr));
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(p), 0, nr, 0x00020000);
+  }
   __device__ uint32_t load(uint32_t voff, uint32_t soff) const {
     return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
   }
@@ -215,14 +240,32 @@ struct GBuf {
   __device__ void store(uint32_t v, uint32_t voff, uint32_t soff) const {
     __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 2);
   }
+  // 16-byte forms (four adjacent columns per lane)
+  using V4 = __attribute__((ext_vector_type(4))) unsigned int;
+  __device__ U4 load4(uint32_t voff, uint32_t soff) const {
+    return __builtin_bit_cast(U4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  }
+  __device__ void store4(const U4& v, uint32_t voff, uint32_t soff) const {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, v), r, voff, soff, 2);
+  }
 #else
   char* b;
-  explicit GBuf(const uint32_t* p) : b(reinterpret_cast<char*>(const_cast<uint32_t*>(p))) {}
-  uint32_t load(uint32_t voff, uint32_t soff) const {
-    return *reinterpret_cast<const uint32_t*>(b + ((size_t)voff + soff));
+  ZHD explicit GBuf(const uint32_t* p) : b(reinterpret_cast<char*>(const_cast<uint32_t*>(p))) {}
+  ZHD GBuf(const uint32_t* p, bool live)
+      : b(live ? reinterpret_cast<char*>(const_cast<uint32_t*>(p)) : nullptr) {}
+  ZHD uint32_t load(uint32_t voff, uint32_t soff) const {
+    return b ? *reinterpret_cast<const uint32_t*>(b + ((size_t)voff + soff)) : 0u;
   }
-  void store(uint32_t v, uint32_t voff, uint32_t soff) const {
-    *reinterpret_cast<uint32_t*>(b + ((size_t)voff + soff)) = v;
+  ZHD void store(uint32_t v, uint32_t voff, uint32_t soff) const {
+    if (b) *reinterpret_cast<uint32_t*>(b + ((size_t)voff + soff)) = v;
+  }
+  ZHD U4 load4(uint32_t voff, uint32_t soff) const {
+    U4 v{0, 0, 0, 0};
+    if (b) v = *reinterpret_cast<const U4*>(b + ((size_t)voff + soff));
+    return v;
+  }
+  ZHD void store4(const U4& v, uint32_t voff, uint32_t soff) const {
+    if (b) *reinterpret_cast<U4*>(b + ((size_t)voff + soff)) = v;
   }
 #endif
 };

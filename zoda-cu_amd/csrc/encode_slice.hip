@@ -1,0 +1,252 @@
+// Slice-pipelined encode of side 2048 on gfx950 (see slice_core.hpp for the
+// transform): one persistent 512-thread work-group per CU, 32-column tiles,
+// the middle of each tile's transform held in registers.
+//
+// Per CU the tiles run as a pipeline: eight steps in which the front of tile
+// i (P1, P2) and the back of tile i - 1 (P2', P1') share each step's LDS
+// exchanges, then the middle passes of tile i, which touch no memory.  Every
+// global access is a 16-byte raw-buffer access (eight lanes per 128-byte row
+// segment): the step's input rows arrive in registers one step ahead, go out
+// again unchanged as the even output rows and into an LDS staging buffer that
+// P1 reads; P1' leaves its odd rows in a second staging buffer that the next
+// step stores.  A dword access per lane would take four times as many
+// vector-memory instructions, and the CU's address unit, busy with every
+// 128-byte row segment, then stalls the issuing waves.
+#include "common.hpp"
+#include "slice_core.hpp"
+
+namespace zoda {
+
+namespace {
+
+using PL = slice::Plan2048;
+using slice::MulK;
+using slice::Tw2;
+
+// Every wave's LDS accesses are complete, then the work-group meets.  Only the
+// LDS counter is waited for: global loads and stores stay in flight across
+// the barrier (a __syncthreads() fence would wait for every store as well).
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+struct Ctx {
+  slice::Tid t;
+  slice::Offs<PL> o;
+  int wu;  // the wave index, uniform
+  int32_t* lds;
+  const Tw2* t1w;
+  const double* t2;
+  const double* t3;
+  MulK k;
+};
+
+// Buffers of one step's memory work.  Every global load and store is
+// unconditional: an inactive part (no front tile in the last iteration, no
+// back tile in the first, no tile after the last) addresses a zero-record
+// buffer instead (GBuf(p, false)), so no memory instruction sits behind a
+// branch and the compiler's counts of outstanding memory operations stay
+// exact -- a wave waits for the loads it needs, never for the stores issued
+// after them (loads and stores share one in-order counter on gfx9).
+struct Bufs {
+  GBuf ibf, obf;  // front tile: input, output
+  GBuf ibn, obn;  // the next front tile
+  GBuf obb;       // back tile: output
+  GBuf obp;       // the tile of the odd rows left in the staging by the previous step
+};
+
+// Step SP (two phases):
+//   phase 1: P1 (IN -> A), P2' (held -> B), the previous step's odd rows out
+//   phase 2: P2 (A -> held), P1' (B -> ODD); the next step's input rows (in
+//            registers since one step) to IN and to the even output rows,
+//            and the loads of the step after
+// MF (memory first) puts each phase's memory work before its arithmetic
+// instead of after it.  The two waves that share a SIMD (w and w + 4) take
+// opposite orders, so that while one of them waits for the address unit to
+// take its loads and stores, the other keeps the SIMD busy.
+template <bool EVM, int SP, bool MF>
+__device__ __forceinline__ void step(const Ctx& c, uint32_t c4, bool front, bool back, const Bufs& b,
+                                     U4 (&ld)[PL::QI], int32_t (&held)[PL::RB][PL::RC]) {
+  // odd rows of step SP - 1 (of the previous iteration's last step for SP = 0)
+  auto odd = [&] {
+    slice::store_odd<PL>(c.t, SP == 0 ? b.obp : b.obb, c.o, SP == 0 ? PL::STEPS - 1 : SP - 1, c4, c.lds);
+  };
+  // rows of step SP + 1 (the next tile's step 0 after the last step) staged,
+  // then the loads of step SP + 2
+  auto stage = [&] {
+    if constexpr (SP + 1 < PL::STEPS)
+      slice::stage_step<PL, !EVM>(c.t, b.obf, c.o, SP + 1, c4, ld, c.lds);
+    else
+      slice::stage_step<PL, !EVM>(c.t, b.obn, c.o, 0, c4, ld, c.lds);
+    if constexpr (SP + 2 < PL::STEPS)
+      slice::load_step<PL>(b.ibf, c.o, c.wu, SP + 2, c4, ld);
+    else
+      slice::load_step<PL>(b.ibn, c.o, c.wu, SP + 2 - PL::STEPS, c4, ld);
+  };
+  if constexpr (MF) {
+    odd();
+    group_fence();
+  }
+  if (front) slice::p1<PL>(c.t, c.lds, c.t1w, c.k);
+  if (back) slice::p2b<PL, SP>(c.t, held, c.lds, c.t3, c.k);
+  if constexpr (!MF) {
+    group_fence();
+    odd();
+  }
+  lds_barrier();
+  if constexpr (MF) {
+    stage();
+    group_fence();
+  }
+  if (front) slice::p2<PL, SP>(c.t, c.lds, c.t2, held, c.k);
+  if (back) slice::p1b<PL>(c.t, c.lds, c.k);
+  if constexpr (!MF) {
+    group_fence();
+    stage();
+  }
+  lds_barrier();
+  if constexpr (SP + 1 < PL::STEPS) step<EVM, SP + 1, MF>(c, c4, front, back, b, ld, held);
+}
+
+// The middle passes of the front tile; with EVM its even rows are copied
+// (read again, 16 bytes a lane) between them, so that the CU keeps memory
+// requests in flight through the passes that have no memory work of their own.
+template <bool EVM, int KB>
+__device__ __forceinline__ void mid(const Ctx& c, uint32_t c4, const Bufs& b, int32_t (&held)[PL::RB][PL::RC]) {
+  U4 ev[PL::QI];
+  if constexpr (EVM) slice::load_step<PL>(b.ibf, c.o, c.wu, KB, c4, ev);
+  slice::mid_group<PL, KB>(held, c.k);
+  if constexpr (EVM) {
+    group_fence();
+#pragma unroll
+    for (int i = 0; i < PL::QI; i++) b.obf.store4(ev[i], c.o.ev, slice::Offs<PL>::s_out(KB, c.wu, i, c4));
+  }
+  if constexpr (KB + 1 < PL::RB) mid<EVM, KB + 1>(c, c4, b, held);
+}
+
+// The waves of the upper half of the work-group run their steps memory first.
+constexpr bool kSplitOrder = true;
+
+// One wave's whole run over its work-group's tiles in step order MF: the
+// prologue, the pipelined tiles and the epilogue.  Both orders are complete
+// functions, so no register state crosses the branch between them (at such a
+// join the two orders' register assignments would have to meet, and spill).
+template <bool EVM, bool MF>
+__device__ __forceinline__ void run_tiles(const uint32_t* in, uint32_t* out, uint32_t cols,
+                                          uint32_t ntiles, const Ctx& c, int tid, int32_t* lds) {
+  const uint32_t b = blockIdx.x, g = gridDim.x;
+  const uint32_t mine = b < ntiles ? (ntiles - 1 - b) / g + 1 : 0;
+  auto tile = [&](uint32_t i) { return xcd_tile(b + i * g, ntiles); };
+  int32_t held[PL::RB][PL::RC];
+#pragma unroll
+  for (int i = 0; i < PL::RB; i++) {
+#pragma unroll
+    for (int j = 0; j < PL::RC; j++) held[i][j] = 0;
+  }
+  // prologue: step 0 of the first tile into IN and its even rows; step 1's loads
+  U4 ld[PL::QI];
+  {
+    const bool live = mine > 0;
+    const uint32_t t0 = live ? tile(0) : 0u;
+    const GBuf ib(in + (size_t)t0 * PL::W, live), ob(out + (size_t)t0 * PL::W, live);
+    slice::load_step<PL>(ib, c.o, c.wu, 0, cols * 4u, ld);
+    slice::stage_step<PL, !EVM>(c.t, ob, c.o, 0, cols * 4u, ld, lds);
+    slice::load_step<PL>(ib, c.o, c.wu, 1, cols * 4u, ld);
+  }
+  lds_barrier();  // the LDS tables and the first IN
+  uint32_t tp = 0;
+  for (uint32_t it = 0; it <= mine; it++) {
+    const bool front = it < mine, back = it > 0, next = it + 1 < mine, prev = it > 1;
+    const uint32_t tf = front ? tile(it) : 0u;
+    const uint32_t tb = back ? tile(it - 1) : 0u;
+    const uint32_t tn = next ? tile(it + 1) : 0u;
+    const Bufs bf{GBuf(in + (size_t)tf * PL::W, front), GBuf(out + (size_t)tf * PL::W, front),
+                  GBuf(in + (size_t)tn * PL::W, next),  GBuf(out + (size_t)tn * PL::W, next),
+                  GBuf(out + (size_t)tb * PL::W, back), GBuf(out + (size_t)tp * PL::W, prev)};
+    uint32_t c4 = cols * 4u;
+    asm volatile("" : "+s"(c4));
+    // the thread's coordinates and addresses are rebuilt from an opaque copy of
+    // its index in every iteration: hoisted out of the loop they outnumber the
+    // free registers and spill
+    int ti = tid;
+    asm volatile("" : "+v"(ti));
+    const slice::Tid tt(ti);
+    const Ctx ci{tt, slice::Offs<PL>(tt, cols), c.wu, lds, c.t1w, c.t2, c.t3, c.k};
+    step<EVM, 0, MF>(ci, c4, front, back, bf, ld, held);
+    if (front) mid<EVM, 0>(ci, c4, bf, held);
+    tp = tb;
+  }
+  // the odd rows of the last tile's last step
+  const GBuf ob(out + (size_t)tp * PL::W, mine > 0);
+  slice::store_odd<PL>(c.t, ob, c.o, PL::STEPS - 1, cols * 4u, lds);
+}
+
+template <bool EVM>
+__global__ __launch_bounds__(PL::NT) void encode_slice(const uint32_t* __restrict__ in,
+                                                       uint32_t* __restrict__ out, uint32_t cols,
+                                                       uint32_t ntiles, const Tw2* __restrict__ t1,
+                                                       const double* __restrict__ t2g,
+                                                       const double* __restrict__ t3g) {
+  extern __shared__ int32_t lds[];
+  const int tid = (int)threadIdx.x;
+  double* t2 = reinterpret_cast<double*>(lds + 4 * PL::XWORDS);
+  double* t3 = t2 + PL::T2_WORDS;
+  for (int i = tid; i < PL::T2_WORDS; i += PL::NT) t2[i] = t2g[i];
+  for (int i = tid; i < PL::T3_WORDS; i += PL::NT) t3[i] = t3g[i];
+  const slice::Tid t(tid);
+  const int wu = __builtin_amdgcn_readfirstlane(t.w);
+  const Ctx c{t, slice::Offs<PL>(t, cols), wu, lds, t1 + wu * PL::RA, t2, t3, slice::make_mulk()};
+  if (kSplitOrder && wu >= PL::NT / 128)
+    run_tiles<EVM, true>(in, out, cols, ntiles, c, tid, lds);
+  else
+    run_tiles<EVM, false>(in, out, cols, ntiles, c, tid, lds);
+}
+
+}  // namespace
+
+bool slice_fits(uint32_t k, uint32_t cols, int layout) {
+  return k == (uint32_t)PL::S && layout == ZODA_LAYOUT_ROW && cols > 0 && cols % PL::W == 0 &&
+         ((uint64_t)(2 * PL::S - 1) * cols + PL::W) * 4 < (1ull << 32);
+}
+
+static int get_slice_tables(DeviceState* st, const Tw2** t1, const double** t2, const double** t3) {
+  std::lock_guard<std::recursive_mutex> lock(st->mu);
+  constexpr size_t bytes = PL::T1_PAIRS * sizeof(Tw2) + (PL::T2_WORDS + PL::T3_WORDS) * sizeof(double);
+  if (!st->slice_tables) {
+    std::vector<char> host(bytes);
+    Tw2* h1 = reinterpret_cast<Tw2*>(host.data());
+    double* h2 = reinterpret_cast<double*>(h1 + PL::T1_PAIRS);
+    slice::build_slice_tables<PL>(h1, h2, h2 + PL::T2_WORDS);
+    void* d = nullptr;
+    ZODA_HIP(hipMalloc(&d, bytes));
+    hipError_t e = hipMemcpy(d, host.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(d);
+      return fail((int)e, "copying slice tables failed: %s", hipGetErrorString(e));
+    }
+    st->slice_tables = d;
+  }
+  *t1 = static_cast<const Tw2*>(st->slice_tables);
+  *t2 = reinterpret_cast<const double*>(*t1 + PL::T1_PAIRS);
+  *t3 = *t2 + PL::T2_WORDS;
+  return 0;
+}
+
+int launch_slice(DeviceState* st, const uint32_t* d_in, uint32_t cols, uint32_t* d_out,
+                 hipStream_t stream, bool dry) {
+  const Tw2* t1 = nullptr;
+  const double *t2 = nullptr, *t3 = nullptr;
+  ZODA_TRY(get_slice_tables(st, &t1, &t2, &t3));
+  const void* fn = (const void*)encode_slice<PL::EVEN_IN_MID>;
+  uint32_t grid = 0;
+  ZODA_TRY(kernel_setup(st, fn, PL::NT, PL::LDS_BYTES, true, &grid));
+  if (dry) return 0;
+  uint32_t ntiles = cols / PL::W;
+  if (grid > ntiles) grid = ntiles;
+  void* args[] = {(void*)&d_in, (void*)&d_out, (void*)&cols, (void*)&ntiles,
+                  (void*)&t1,   (void*)&t2,    (void*)&t3};
+  ZODA_HIP(hipLaunchKernel(fn, dim3(grid), dim3(PL::NT), args, PL::LDS_BYTES, stream));
+  return 0;
+}
+
+}  // namespace zoda
