@@ -70,6 +70,39 @@ def test_argument_errors_do_not_touch_the_device():
         zoda_hip.ntt_gpu(buf[:60], 8, 16)
 
 
+def test_runtime_failures_return_the_hip_status():
+    """Without a visible device every entry point that reaches the HIP runtime
+    returns the runtime's positive hipError_t (hipErrorNoDevice) and a message,
+    never a ZODA_E* code and never an abort."""
+    import ctypes
+    import zoda_hip
+    if zoda_hip.gpu_available():
+        pytest.skip("a device is visible")
+    lib = zoda_hip.lib
+    buf = np.zeros(1 << 12, dtype=np.uint32)
+    roots = np.ones(64, dtype=np.uint32)
+    p, r = buf.ctypes.data, roots.ctypes.data
+    rows = (ctypes.c_uint32 * 64)(*range(64))
+    calls = {
+        "ntt": lambda: lib.zoda_hip_ntt(p, r, 64, 1, 64),
+        "intt": lambda: lib.zoda_hip_intt(p, r, 64, 1, 64),
+        "ntt_mod": lambda: lib.zoda_hip_ntt_mod(p, r, 64, 1, 64, 257, 0),
+        "encode_host": lambda: lib.zoda_hip_encode_host(p, 16, 16, 32, p, 0),
+        "encode_device": lambda: lib.zoda_hip_encode_device(p, 16, 16, 32, p, 0, None),
+        "ntt_device": lambda: lib.zoda_hip_ntt_device(p, 64, 1, 64, 0, None),
+        "fill_uniform": lambda: lib.zoda_hip_fill_uniform(p, 64, 1, None),
+        "rlc_coefficients": lambda: lib.zoda_hip_rlc_coefficients(b"ab" * 32, 64, 8, p, None),
+        "rlc_rows": lambda: lib.zoda_hip_rlc_rows(p, 8, 8, None, 8, p, p, None),
+        "decode_device": lambda: lib.zoda_hip_decode_device(p, 64, 8, rows, 64, p, None),
+        "autotune": lambda: lib.zoda_hip_autotune((ctypes.c_uint32 * 1)(64), 1, 1, None),
+        "server_create": lambda: lib.zoda_server_create(1 << 30, 2, ctypes.byref(ctypes.c_void_p())),
+    }
+    for name, call in calls.items():
+        rc = call()
+        assert rc > 0, (name, rc)
+        assert zoda_hip.last_error(), name
+
+
 def test_uniform_generator_host_restatement():
     import zoda_hip
     a = zoda_hip.uniform_numpy(100000, 7)

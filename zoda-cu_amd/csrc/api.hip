@@ -51,9 +51,9 @@ int get_buffer(DeviceState* st, int slot, size_t bytes, void** out) {
     if (st->buffers[slot]) ZODA_HIP(hipFree(st->buffers[slot]));
     st->buffers[slot] = nullptr;
     st->buffer_bytes[slot] = 0;
-    if (hipMalloc(&st->buffers[slot], bytes) != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(ZODA_ENOMEM, "device allocation of %zu bytes failed", bytes);
+    if (const hipError_t ea = hipMalloc(&st->buffers[slot], bytes)) {
+      st->buffers[slot] = nullptr;
+      return alloc_fail(ea, "device staging buffer");
     }
     st->buffer_bytes[slot] = bytes;
   }
@@ -75,14 +75,10 @@ static int upload_mont(const std::vector<uint32_t>& fwd, const std::vector<uint3
                        MontTables* out) {
   MontTables t;
   const size_t bytes = fwd.size() * sizeof(uint32_t);
-  if (hipMalloc((void**)&t.fwd, bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(ZODA_ENOMEM, "twiddle table allocation failed");
-  }
-  if (hipMalloc((void**)&t.inv, bytes) != hipSuccess) {
-    (void)hipGetLastError();
+  if (const hipError_t ea = hipMalloc((void**)&t.fwd, bytes)) return alloc_fail(ea, "twiddle table allocation");
+  if (const hipError_t ea = hipMalloc((void**)&t.inv, bytes)) {
     (void)hipFree(t.fwd);
-    return fail(ZODA_ENOMEM, "twiddle table allocation failed");
+    return alloc_fail(ea, "twiddle table allocation");
   }
   ZODA_HIP(hipMemcpy(t.fwd, fwd.data(), bytes, hipMemcpyHostToDevice));
   ZODA_HIP(hipMemcpy(t.inv, inv.data(), bytes, hipMemcpyHostToDevice));

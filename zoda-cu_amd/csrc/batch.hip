@@ -337,21 +337,20 @@ static int build_plan(const zoda_square* squares, uint32_t count, bool graph, bo
     free_plan(p);
     return rc;
   };
-  bool made = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) == hipSuccess;
-  for (int b = 0; b < BATCH_BRANCHES && made; b++)
-    made = hipStreamCreateWithFlags(&p->side[b], hipStreamNonBlocking) == hipSuccess &&
-           hipEventCreateWithFlags(&p->join[b], hipEventDisableTiming) == hipSuccess;
-  if (!made) {
-    (void)hipGetLastError();
-    return bail(fail(ZODA_ENOMEM, "batch plan stream/event creation failed"));
+  int made = 0;
+  hip_note(made, hipEventCreateWithFlags(&p->fork, hipEventDisableTiming), "batch plan event creation");
+  for (int b = 0; b < BATCH_BRANCHES && !made; b++) {
+    hip_note(made, hipStreamCreateWithFlags(&p->side[b], hipStreamNonBlocking), "batch plan stream creation");
+    if (!made) hip_note(made, hipEventCreateWithFlags(&p->join[b], hipEventDisableTiming), "batch plan event creation");
   }
+  if (made) return bail(made);
   // Zero-sized arrays still get an allocation so the kernel arguments are valid.
-  if (hipMalloc((void**)&p->d_squares, std::max<size_t>(1, descs.size()) * sizeof(SquareDesc)) != hipSuccess ||
-      hipMalloc((void**)&p->d_items, std::max<size_t>(1, items.size()) * sizeof(BatchItem)) != hipSuccess ||
-      hipMalloc((void**)&p->d_tabs, tabs.size() * sizeof(EncodeTabs)) != hipSuccess ||
-      hipMalloc((void**)&p->d_counter, sizeof(uint32_t)) != hipSuccess) {
-    (void)hipGetLastError();
-    return bail(fail(ZODA_ENOMEM, "batch plan allocation failed"));
+  {
+    hipError_t ea = hipMalloc((void**)&p->d_squares, std::max<size_t>(1, descs.size()) * sizeof(SquareDesc));
+    if (ea == hipSuccess) ea = hipMalloc((void**)&p->d_items, std::max<size_t>(1, items.size()) * sizeof(BatchItem));
+    if (ea == hipSuccess) ea = hipMalloc((void**)&p->d_tabs, tabs.size() * sizeof(EncodeTabs));
+    if (ea == hipSuccess) ea = hipMalloc((void**)&p->d_counter, sizeof(uint32_t));
+    if (ea != hipSuccess) return bail(alloc_fail(ea, "batch plan allocation"));
   }
   hipError_t e = hipSuccess;
   if (!descs.empty())
@@ -461,11 +460,11 @@ extern "C" int zoda_hip_autotune(const uint32_t* sides, uint32_t count, int iter
     const uint32_t L = log2_ceil(side);
     const uint64_t cols = 128ull * side;
     uint32_t *din = nullptr, *dout = nullptr;
-    if (hipMalloc((void**)&din, side * cols * 4) != hipSuccess ||
-        hipMalloc((void**)&dout, 2 * side * cols * 4) != hipSuccess) {
-      (void)hipGetLastError();
+    hipError_t ea = hipMalloc((void**)&din, side * cols * 4);
+    if (ea == hipSuccess) ea = hipMalloc((void**)&dout, 2 * side * cols * 4);
+    if (ea != hipSuccess) {
       (void)hipFree(din);
-      rc = fail(ZODA_ENOMEM, "autotune buffers for side %u", side);
+      rc = alloc_fail(ea, "autotune buffers");
       break;
     }
     rc = launch_fill_uniform(din, side * cols, side, s);
@@ -482,7 +481,7 @@ extern "C" int zoda_hip_autotune(const uint32_t* sides, uint32_t count, int iter
         hipEventRecord(e0, s);
         rc = launch_batch_direct(p, s);
         hipEventRecord(e1, s);
-        if (hipEventSynchronize(e1) != hipSuccess) rc = fail(ZODA_ESTATE, "autotune launch failed");
+        hip_note(rc, hipEventSynchronize(e1), "autotune launch");
         float ms = 0;
         hipEventElapsedTime(&ms, e0, e1);
         if (it > 0) vbest = std::min(vbest, ms * 1000.0f);  // first launch warms up

@@ -20,6 +20,23 @@ namespace zoda {
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
 
+// Records a failed HIP call in rc as its hipError_t (positive), unless rc
+// already holds an earlier error.  The call is evaluated either way, so
+// clean-up calls still run after a failure.
+inline void hip_note(int& rc, hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  (void)hipGetLastError();
+  if (rc == 0) rc = fail((int)e, "%s: %s", what, hipGetErrorString(e));
+}
+
+// Status of a failed allocation: ZODA_ENOMEM when the device is out of
+// memory, the positive hipError_t for any other cause (no device, ...).
+inline int alloc_fail(hipError_t e, const char* what) {
+  (void)hipGetLastError();
+  if (e == hipErrorOutOfMemory) return fail(ZODA_ENOMEM, "%s: out of device memory", what);
+  return fail((int)e, "%s: %s", what, hipGetErrorString(e));
+}
+
 // Evaluates a HIP call; on failure records the error and returns it (positive).
 #define ZODA_HIP(call)                                                            \
   do {                                                                            \

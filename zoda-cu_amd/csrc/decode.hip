@@ -281,10 +281,8 @@ int get_decode_tables(DeviceState* st, const DecodeTabs** out) {
     const DecodeTableImage im = build_decode_tables<PL>();
     if (im.packed.empty()) return fail(ZODA_ESTATE, "decoder table layout mismatch for n = 2^%d", L);
     DecodeTableSet set;
-    if (hipMalloc((void**)&set.packed, im.packed.size() * sizeof(uint32_t)) != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(ZODA_ENOMEM, "decoder table allocation failed");
-    }
+    if (const hipError_t ea = hipMalloc((void**)&set.packed, im.packed.size() * sizeof(uint32_t)))
+      return alloc_fail(ea, "decoder table allocation");
     const hipError_t e =
         hipMemcpy(set.packed, im.packed.data(), im.packed.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -426,19 +424,20 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
   host.insert(host.end(), ep_m.begin(), ep_m.end());
   host.insert(host.end(), rows.begin(), rows.end());
   int rc = 0;
-  if (hipMemcpy(d_blk, host.data(), up_words * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(d_known, known.data(), n, hipMemcpyHostToDevice) != hipSuccess)
-    rc = fail(ZODA_ESTATE, "decode vectors upload failed");
+  hip_note(rc, hipMemcpy(d_blk, host.data(), up_words * 4, hipMemcpyHostToDevice), "decode vectors upload");
+  if (!rc) hip_note(rc, hipMemcpy(d_known, known.data(), n, hipMemcpyHostToDevice), "decode vectors upload");
   // Scratch: the k known rows per column (FP64 path) or all n rows.
   const uint32_t t_rows = fp64 ? k : n;
-  if (!rc && hipMallocAsync((void**)&T, (size_t)cols * t_rows * sizeof(uint32_t), s) != hipSuccess) {
-    T = nullptr;
-    rc = fail(ZODA_ENOMEM, "decode scratch allocation failed");
+  if (!rc) {
+    if (const hipError_t ea = hipMallocAsync((void**)&T, (size_t)cols * t_rows * sizeof(uint32_t), s)) {
+      T = nullptr;
+      rc = alloc_fail(ea, "decode scratch allocation");
+    }
   }
   const size_t lds = (size_t)n * sizeof(uint32_t);
-  if (!rc && !fp64 && lds > 65536 &&
-      hipFuncSetAttribute((const void*)dec_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-    rc = fail(ZODA_ESTATE, "raising the decoder's LDS limit failed");
+  if (!rc && !fp64 && lds > 65536)
+    hip_note(rc, hipFuncSetAttribute((const void*)dec_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+             "raising the decoder's LDS limit");
   if (!rc) {
     hipLaunchKernelGGL(dec_prep, dim3((2 * n + 255) / 256), dim3(256), 0, s, d_x, d_known, d_ep, ne, log_n,
                        to_mont_host(DEC_COSET), fp64 ? 1 : 0, to_mont_host(n), d_zk, d_zinv);
@@ -460,11 +459,11 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
     }
     hipLaunchKernelGGL(dec_scatter, dim3((k + TT - 1) / TT, (cols + TT - 1) / TT), dim3(TT, SCATTER_NY), 0, s, T, t_rows, k,
                        cols, fp64 ? d_known : nullptr, d_data);
-    if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "decode launch failed");
+    hip_note(rc, hipGetLastError(), "decode launch");
   }
-  if (T && hipFreeAsync(T, s) != hipSuccess && !rc) rc = fail(ZODA_ESTATE, "hipFreeAsync failed");
+  if (T) hip_note(rc, hipFreeAsync(T, s), "hipFreeAsync");
   // d_blk is read by the kernels just queued: free it once they are done.
-  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(ZODA_ESTATE, "decode failed");
+  hip_note(rc, hipStreamSynchronize(s), "decode");
   (void)hipFree(d_blk);
   return rc;
 }

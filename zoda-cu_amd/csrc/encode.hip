@@ -106,11 +106,11 @@ int get_encode_tables(DeviceState* st, int L, int r1, int r2, int rd, const Enco
     std::vector<Tw> host = build_encode_tables(L, r1, r2, rd, &off);
     PackedTables pk = pack_u32(host, off);
     EncodeTableSet set;
-    if (hipMalloc((void**)&set.base, host.size() * sizeof(Tw)) != hipSuccess ||
-        hipMalloc((void**)&set.packed, pk.words.size() * sizeof(uint32_t)) != hipSuccess) {
-      (void)hipGetLastError();
+    hipError_t ea = hipMalloc((void**)&set.base, host.size() * sizeof(Tw));
+    if (ea == hipSuccess) ea = hipMalloc((void**)&set.packed, pk.words.size() * sizeof(uint32_t));
+    if (ea != hipSuccess) {
       (void)hipFree(set.base);
-      return fail(ZODA_ENOMEM, "encode table allocation failed");
+      return alloc_fail(ea, "encode table allocation");
     }
     hipError_t e = hipMemcpy(set.base, host.data(), host.size() * sizeof(Tw), hipMemcpyHostToDevice);
     if (e == hipSuccess)

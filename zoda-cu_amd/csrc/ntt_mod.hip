@@ -80,27 +80,26 @@ extern "C" int zoda_hip_ntt_mod(uint32_t* values, const uint32_t* roots, uint32_
   uint32_t *d_v = nullptr, *d_r = nullptr;
   int rc = 0;
   const size_t pitch = (size_t)stride * 4, row = (size_t)n * 4;
-  if (hipMalloc((void**)&d_v, (size_t)polys * row) != hipSuccess || hipMalloc((void**)&d_r, row) != hipSuccess) {
-    (void)hipGetLastError();
-    rc = fail(ZODA_ENOMEM, "device buffers for %u polynomials", polys);
-  }
-  if (!rc && hipMemcpy(d_r, r.data(), row, hipMemcpyHostToDevice) != hipSuccess)
-    rc = fail(ZODA_ESTATE, "copying the roots failed");
-  if (!rc && hipMemcpy2DAsync(d_v, row, values, pitch, row, polys, hipMemcpyHostToDevice, s) != hipSuccess)
-    rc = fail(ZODA_ESTATE, "copying the values failed");
+  if (const hipError_t ea = hipMalloc((void**)&d_v, (size_t)polys * row)) rc = alloc_fail(ea, "device buffer of the values");
+  if (!rc)
+    if (const hipError_t ea = hipMalloc((void**)&d_r, row)) rc = alloc_fail(ea, "device buffer of the roots");
+  if (!rc) hip_note(rc, hipMemcpy(d_r, r.data(), row, hipMemcpyHostToDevice), "copying the roots");
+  if (!rc)
+    hip_note(rc, hipMemcpy2DAsync(d_v, row, values, pitch, row, polys, hipMemcpyHostToDevice, s), "copying the values");
   if (!rc) {
     uint32_t done = 0;
     while (!rc && done < polys) {  // grid x is limited: launch in slices
       const uint32_t w = std::min<uint32_t>(polys - done, 1u << 20);
       hipLaunchKernelGGL(ntt_mod_kernel, dim3(w), dim3(MOD_NT), row, s, d_v + (size_t)done * n, d_r,
                          log2_ceil(n), n, modulus, n_inv, inverse ? 1 : 0);
-      if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "transform launch failed");
+      hip_note(rc, hipGetLastError(), "transform launch");
       done += w;
     }
   }
-  if (!rc && hipMemcpy2DAsync(values, pitch, d_v, row, row, polys, hipMemcpyDeviceToHost, s) != hipSuccess)
-    rc = fail(ZODA_ESTATE, "copying the values back failed");
-  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(ZODA_ESTATE, "transform failed");
+  if (!rc)
+    hip_note(rc, hipMemcpy2DAsync(values, pitch, d_v, row, row, polys, hipMemcpyDeviceToHost, s),
+             "copying the values back");
+  hip_note(rc, hipStreamSynchronize(s), "transform");
   (void)hipFree(d_v);
   (void)hipFree(d_r);
   (void)hipStreamDestroy(s);

@@ -254,23 +254,23 @@ extern "C" int zoda_hip_rlc_coefficients(const char* data_root, uint32_t root_le
     for (int i = 0; i < 8; i++) mid.h[i] = h.h[i];
     hipLaunchKernelGGL(rlc_coeff_mid_kernel, dim3((cols + 255) / 256), dim3(256), 0, s, mid,
                        (uint64_t)root_len * 8 + 64, cols, d_coeffs);
-    if (hipGetLastError() != hipSuccess) return fail(ZODA_ESTATE, "coefficient kernel launch failed");
-    if (hipStreamSynchronize(s) != hipSuccess) return fail(ZODA_ESTATE, "coefficient derivation failed");
-    return 0;
+    int rc = 0;
+    hip_note(rc, hipGetLastError(), "coefficient kernel launch");
+    hip_note(rc, hipStreamSynchronize(s), "coefficient derivation");
+    return rc;
   }
   uint8_t* d_root = nullptr;
   ZODA_HIP(hipMallocAsync((void**)&d_root, root_len ? root_len : 1, s));
   int rc = 0;
-  if (root_len && hipMemcpyAsync(d_root, data_root, root_len, hipMemcpyHostToDevice, s) != hipSuccess)
-    rc = fail(ZODA_ESTATE, "copying the data root failed");
+  if (root_len) hip_note(rc, hipMemcpyAsync(d_root, data_root, root_len, hipMemcpyHostToDevice, s), "copying the data root");
   if (!rc) {
     hipLaunchKernelGGL(rlc_coeff_kernel, dim3((cols + 255) / 256), dim3(256), 0, s, d_root, root_len, cols,
                        d_coeffs);
-    if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "coefficient kernel launch failed");
+    hip_note(rc, hipGetLastError(), "coefficient kernel launch");
   }
   // The copy reads the caller's (pageable) string: finish before returning.
-  if (hipFreeAsync(d_root, s) != hipSuccess && !rc) rc = fail(ZODA_ESTATE, "hipFreeAsync failed");
-  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(ZODA_ESTATE, "coefficient derivation failed");
+  hip_note(rc, hipFreeAsync(d_root, s), "hipFreeAsync");
+  hip_note(rc, hipStreamSynchronize(s), "coefficient derivation");
   return rc;
 }
 
@@ -292,13 +292,13 @@ extern "C" int zoda_hip_rlc_rows(const uint32_t* d_square, uint32_t rows, uint32
   hipLaunchKernelGGL(rlc_rows_kernel, dim3(nblk, nrb), dim3(RLC_NT), 0, s, d_square, rows, cols, d_row_index,
                      count, reinterpret_cast<const uint4*>(d_coeffs), partial);
   int rc = 0;
-  if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "RLC kernel launch failed");
+  hip_note(rc, hipGetLastError(), "RLC kernel launch");
   if (!rc) {
     hipLaunchKernelGGL(rlc_sum_kernel, dim3((count * 4 + 255) / 256), dim3(256), 0, s, partial, count, nblk,
                        d_out);
-    if (hipGetLastError() != hipSuccess) rc = fail(ZODA_ESTATE, "RLC reduction launch failed");
+    hip_note(rc, hipGetLastError(), "RLC reduction launch");
   }
-  if (hipFreeAsync(partial, s) != hipSuccess && !rc) rc = fail(ZODA_ESTATE, "hipFreeAsync failed");
+  hip_note(rc, hipFreeAsync(partial, s), "hipFreeAsync");
   return rc;
 }
 

@@ -82,10 +82,8 @@ class Server {
     for (Slot& s : slots_) {
       ZODA_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
       ZODA_HIP(hipEventCreate(&s.done));
-      if (hipMalloc((void**)&s.d_counter, sizeof(uint32_t)) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(ZODA_ENOMEM, "serving slot allocation failed");
-      }
+      if (const hipError_t ea = hipMalloc((void**)&s.d_counter, sizeof(uint32_t)))
+        return alloc_fail(ea, "serving slot allocation");
       ZODA_HIP(hipMemset(s.d_counter, 0, sizeof(uint32_t)));
     }
     // The slot streams do not wait on the null stream: the zero fills must
@@ -460,10 +458,10 @@ extern "C" int zoda_hip_serve_trace(const zoda_square* reqs, const double* arriv
   std::vector<hipEvent_t> ev(count + 1, nullptr);
   int rc = 0;
   for (auto& e : ev)
-    if (!rc && hipEventCreate(&e) != hipSuccess) rc = fail(ZODA_ENOMEM, "event creation failed");
+    if (!rc) hip_note(rc, hipEventCreate(&e), "event creation");
   double start_host = 0, t0 = 0;
-  if (!rc && (hipEventRecord(ev[count], s) != hipSuccess || hipEventSynchronize(ev[count]) != hipSuccess))
-    rc = fail(ZODA_ESTATE, "start event failed");
+  if (!rc) hip_note(rc, hipEventRecord(ev[count], s), "start event");
+  if (!rc) hip_note(rc, hipEventSynchronize(ev[count]), "start event");
   if (!rc) {
     start_host = now_us();
     t0 = start_host;
@@ -474,17 +472,15 @@ extern "C" int zoda_hip_serve_trace(const zoda_square* reqs, const double* arriv
     }
     const zoda_square& q = reqs[i];
     rc = launch_encode(st, q.in, q.k, q.cols, q.n, q.out, ZODA_LAYOUT_ROW, s);
-    if (!rc && hipEventRecord(ev[i], s) != hipSuccess) rc = fail(ZODA_ESTATE, "event record failed");
+    if (!rc) hip_note(rc, hipEventRecord(ev[i], s), "event record");
     peak = std::max(peak, request_bytes(q));
   }
-  if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(ZODA_ESTATE, "plain loop failed");
+  if (!rc) hip_note(rc, hipStreamSynchronize(s), "plain loop");
   double last = 0;
   for (uint32_t i = 0; i < count && !rc; i++) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, ev[count], ev[i]) != hipSuccess) {
-      rc = fail(ZODA_ESTATE, "event timing failed");
-      break;
-    }
+    hip_note(rc, hipEventElapsedTime(&ms, ev[count], ev[i]), "event timing");
+    if (rc) break;
     const double done = start_host + ms * 1000.0;
     latency_us[i] = done - (t0 + arrival_us[i]);
     last = std::max(last, done);
