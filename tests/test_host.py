@@ -103,6 +103,21 @@ def test_runtime_failures_return_the_hip_status():
         assert zoda_hip.last_error(), name
 
 
+def test_kernel_emulations_under_sanitizers():
+    """Every kernel emulation (fused plans in both layouts, batched plans,
+    persistent and slice kernels, the decoder) rebuilt with AddressSanitizer
+    and UndefinedBehaviorSanitizer and run against the oracle: an LDS or global
+    index outside its buffer, or undefined integer arithmetic, fails the run."""
+    import subprocess
+    sim = os.path.join(os.path.dirname(__file__), "sim")
+    subprocess.run(["make", "-s", "-C", sim, "sanitize"], check=True, capture_output=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
+    res = subprocess.run([os.path.join(sim, "sim_check_san")], capture_output=True, text=True, env=env,
+                         timeout=600)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-4000:]
+    assert "0 mismatches" in res.stdout
+
+
 def test_uniform_generator_host_restatement():
     import zoda_hip
     a = zoda_hip.uniform_numpy(100000, 7)
