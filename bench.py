@@ -48,6 +48,10 @@ def parse():
                     help="requests per served trace (p99 rests on 1 %% of them)")
     ap.add_argument("--no-rlc", action="store_true", help="skip the RLC leg")
     ap.add_argument("--no-decode", action="store_true", help="skip the erasure-decode leg")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the reference's own window: zoda_hip_intt + zoda_hip_ntt on the pinned square")
+    ap.add_argument("--no-col", action="store_true", help="skip the column-major device encode leg")
+    ap.add_argument("--no-4096", action="store_true", help="skip the 4096^2 square leg")
     ap.add_argument("--only-encode", action="store_true",
                     help="time the encode only (no end-to-end, batched, served or CPU legs); for profiling")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -55,6 +59,7 @@ def parse():
     args = ap.parse_args()
     if args.only_encode:
         args.no_e2e = args.no_cpu = args.no_batched = args.no_served = args.no_rlc = args.no_decode = True
+        args.no_dropin = args.no_col = args.no_4096 = True
     return args
 
 
@@ -74,14 +79,21 @@ def traffic_from_profile(side: int):
 
 def cpu_baseline(side: int, ncols: int, seed: int):
     """The oracle (a C restatement of src/ntt/cpu.rs and the encode of
-    src/benchmarks/utils.rs) on `ncols` columns of the same square, all host
-    cores, plus the same sample on one core for a quarter of the columns."""
+    src/benchmarks/utils.rs) on `ncols` columns of the same square over the
+    process's CPU share (at most 16 threads: one GPU's share of the host),
+    plus a quarter of the sample on one core, the form of the reference's own
+    single-threaded CPU encode (src/zoda/babybear.rs:41-44)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle
     import zoda_hip
     data = zoda_hip.uniform_numpy(side * ncols, seed).reshape(side, ncols)
-    threads = min(os.cpu_count() or 1, 16)
+    host_cores = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = host_cores
+    threads = max(1, min(allowed, 16))
     t0 = time.perf_counter()
     oracle.encode(data, threads=threads)
     dt = time.perf_counter() - t0
@@ -97,7 +109,10 @@ def cpu_baseline(side: int, ncols: int, seed: int):
         "cores": threads,
         "kind": "port",
         "sample": f"{ncols} of {128 * side} columns of the {side}^2 square on {threads} threads "
-                  f"({dt:.2f} s); full-square time extrapolated per column",
+                  f"({dt:.2f} s); Gb/s of input data per second, the same unit as the GPU line",
+        "host_cores": host_cores,
+        "cores_allowed": allowed,
+        "threads_note": "threads = the cores this process may run on, capped at 16 (one GPU's share of the host)",
         "single_core_value": round(gbps1, 4),
         "single_core_sample": f"{one} columns on 1 thread ({dt1:.2f} s)",
         "cpu_model": _cpu_model(),
@@ -168,42 +183,62 @@ def batched_vs_plain(zoda_hip, torch, seed: int, reps: int = 7):
     }
 
 
-def served_vs_plain(zoda_hip, torch, seed: int, count: int, rates=(2000, 8000), slots: int = 3):
+def served_vs_plain(zoda_hip, torch, seed: int, count: int, rates=(2000, 8000), slots: int = 3,
+                    checked_per_trace: int = 24):
     """Poisson arrivals of squares with log-uniform sides {64..1024} (512 B
-    shares, device-resident payloads drawn from a pool of 12 buffers per side),
+    shares, device-resident inputs drawn from 12 read-only buffers per side),
     replayed in real time against the continuous-batching server (HBM budget
-    200 GB, 3 launch slots) and against a plain one-square-at-a-time loop."""
+    200 GB, 3 launch slots) and against a plain one-square-at-a-time loop.
+    The server takes each request's output from its device pool at admission
+    and returns it at completion (the plain loop writes one block per size, one
+    request at a time); a seeded sample of requests per trace writes to
+    buffers of its own instead, checked after the trace against the same
+    square encoded alone."""
     import numpy as np
     sides_all = [64, 128, 256, 512, 1024]
+    budget = 200 * 10**9
     pool = {}
     for sd in sides_all:
         pool[sd] = []
         for j in range(12):
             t_in = torch.empty((sd, 128 * sd), dtype=torch.int32, device="cuda")
             zoda_hip.fill_uniform(t_in, seed=seed + 100 * sd + j)
-            t_out = torch.empty((2 * sd, 128 * sd), dtype=torch.int32, device="cuda")
-            pool[sd].append((t_in, t_out))
+            pool[sd].append(t_in)
     torch.cuda.synchronize()
     # Untimed warm-up of both paths with one request of every side, so that
     # neither pays first-use costs (code object loading, twiddle tables) in
     # its timed trace: a running server keeps those resident.
-    warm = [(pool[sd][0][0].data_ptr(), pool[sd][0][1].data_ptr(), sd, 128 * sd, sd) for sd in sides_all]
+    warm = [(pool[sd][0].data_ptr(), 0, sd, 128 * sd, sd) for sd in sides_all]
     for mode in (0, 1):
-        zoda_hip.serve_trace(warm, np.zeros(len(warm)), hbm_budget=200 * 10**9, slots=slots, mode=mode)
+        zoda_hip.serve_trace(warm, np.zeros(len(warm)), hbm_budget=budget, slots=slots, mode=mode)
     out = {}
+    all_ok = True
     for rate in rates:
         rng = np.random.default_rng(seed + rate)
         sides = rng.choice(sides_all, size=count).tolist()
         gaps = rng.exponential(1e6 / rate, size=count)
         arrivals = np.cumsum(gaps) - gaps[0]
-        specs = []
-        for i, sd in enumerate(sides):
-            t_in, t_out = pool[sd][i % len(pool[sd])]
-            specs.append((t_in.data_ptr(), t_out.data_ptr(), sd, 128 * sd, sd))
+        checked = sorted(rng.choice(count, size=min(checked_per_trace, count), replace=False).tolist())
         payload = sum(sd * sd * SHARE_BYTES for sd in sides)
         res = {}
         for mode, name in ((0, "served"), (1, "plain_loop")):
-            lat, st = zoda_hip.serve_trace(specs, arrivals, hbm_budget=200 * 10**9, slots=slots, mode=mode)
+            own = {i: torch.zeros((2 * sides[i], 128 * sides[i]), dtype=torch.int32, device="cuda")
+                   for i in checked}
+            specs = []
+            for i, sd in enumerate(sides):
+                t_in = pool[sd][i % len(pool[sd])]
+                specs.append((t_in.data_ptr(), own[i].data_ptr() if i in own else 0, sd, 128 * sd, sd))
+            torch.cuda.synchronize()
+            lat, st = zoda_hip.serve_trace(specs, arrivals, hbm_budget=budget, slots=slots, mode=mode)
+            ok = True
+            for i in checked:
+                ref = zoda_hip.encode_tensor(pool[sides[i]][i % 12])
+                torch.cuda.synchronize()
+                ok = ok and bool(torch.equal(ref, own[i]))
+                del ref
+            all_ok = all_ok and ok
+            del own
+            torch.cuda.empty_cache()
             res[name] = {
                 "p50_us": round(float(np.percentile(lat, 50)), 1),
                 "p99_us": round(float(np.percentile(lat, 99)), 1),
@@ -211,11 +246,16 @@ def served_vs_plain(zoda_hip, torch, seed: int, count: int, rates=(2000, 8000), 
                 "gbps": round(payload * 8 / 1e9 / (st["makespan_us"] / 1e6), 2),
                 "launches": st["batches"],
                 "max_batch": st["max_batch"],
+                "output_pool_peak_bytes": st["pool_peak_bytes"],
+                "outputs_checked": len(checked),
+                "outputs_match_alone": ok,
             }
         out[f"{rate}_req_per_s"] = res
     return {
         "workload": f"{count} requests, Poisson arrivals, sides log-uniform {{64..1024}}, 512 B shares, "
-                    f"HBM budget 200 GB, {slots} slots; Gb/s = payload over first arrival to last completion",
+                    f"HBM budget 200 GB, {slots} slots; outputs from the server's pool (plain loop: one block "
+                    f"per size); Gb/s = payload over first arrival to last completion",
+        "outputs_checked": all_ok,
         **out,
     }
 
@@ -308,15 +348,183 @@ def decode_leg(zoda_hip, torch, t_in, t_out, seed: int, reps: int = 3):
     }
 
 
+def time_events(torch, fn, steps: int, warmup: int = 1):
+    """HIP-event time of each of `steps` calls of fn on the current stream
+    (after `warmup` untimed calls): (avg ms, p50 ms)."""
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    ms = sorted(a.elapsed_time(b) for a, b in ev)
+    return sum(ms) / len(ms), ms[len(ms) // 2]
+
+
+def roofline_of(alg_bytes: int, ms: float, traffic=None):
+    gbs = alg_bytes / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": traffic, "algorithmic_bytes": alg_bytes,
+            "kernel_ms_avg": round(ms, 4)}
+
+
+def spot_check(torch, t_in, t_out_rows, pick):
+    """Columns `pick` of the encoded square (rows x cols tensor, or a callable
+    returning the 2s x len(pick) block) against the oracle."""
+    import numpy as np
+    import oracle
+    idx = torch.tensor(pick, device=t_in.device)
+    d_s = t_in[:, idx].contiguous().cpu().numpy().view(np.uint32)
+    o_s = t_out_rows(idx)
+    return bool(np.array_equal(o_s, oracle.encode(d_s)))
+
+
+def dropin_leg(zoda_hip, t_in, t_out, reps: int = 5):
+    """The reference's own "GPU xform" window (src/benchmarks/utils.rs:137-145):
+    intt_cuda(square, n_k, stride n_kn) then ntt_cuda(square, n_kn, stride
+    n_kn) on the pinned column-major square (src/ntt/cuda.rs:31-67), here
+    zoda_hip_intt + zoda_hip_ntt.  Host memory in and out (PCIe inside the
+    window, as in the reference), wall clock, median of `reps` after one
+    warm-up; the warm-up's result is checked against the device encode."""
+    import numpy as np
+    side, cols = t_in.shape
+    nkn = 2 * side
+    sq = zoda_hip.PinnedSquare(nkn, cols)
+    try:
+        a = sq.array.reshape(cols, nkn)
+        a[:, :side] = t_in.cpu().numpy().view(np.uint32).T
+        rk, rkn = zoda_hip.roots_of_unity(side), zoda_hip.roots_of_unity(nkn)
+        ptr = sq.array.ctypes.data
+
+        def xform():
+            zoda_hip._check(zoda_hip.lib.zoda_hip_intt(ptr, rk.ctypes.data, side, cols, nkn), "zoda_hip_intt")
+            zoda_hip._check(zoda_hip.lib.zoda_hip_ntt(ptr, rkn.ctypes.data, nkn, cols, nkn), "zoda_hip_ntt")
+
+        xform()
+        pick = [0, 1, cols // 3, cols - 1]
+        ok = all(np.array_equal(a[c], t_out[:, c].cpu().numpy().view(np.uint32)) for c in pick)
+        times = []
+        for _ in range(reps):  # in place: later runs transform the previous result (same work)
+            t0 = time.perf_counter()
+            xform()
+            times.append(time.perf_counter() - t0)
+        times.sort()
+        t = times[len(times) // 2]
+        return {
+            "window": "zoda_hip_intt(n_k, stride n_kn) + zoda_hip_ntt(n_kn, stride n_kn) on the pinned "
+                      f"column-major {cols} x {nkn} square; host memory in and out; wall clock, median of {reps}",
+            "ms_p50": round(t * 1e3, 2),
+            "gbps": round(side * side * SHARE_BYTES * 8 / 1e9 / t, 2),
+            "matches_device_encode": ok,
+        }
+    finally:
+        sq.free()
+
+
+def column_major_leg(zoda_hip, torch, t_in, steps: int):
+    """The device-resident encode in the reference's output layout
+    (ZODA_LAYOUT_COL: out[c * n_kn + r], src/ntt/cuda.rs:31-67)."""
+    side, cols = t_in.shape
+    nkn = 2 * side
+    t_col = torch.empty((cols, nkn), dtype=torch.int32, device=t_in.device)
+    sptr = torch.cuda.current_stream().cuda_stream
+    fn = lambda: zoda_hip.encode_device(t_in.data_ptr(), side, cols, side, t_col.data_ptr(), 1, sptr)
+    avg, p50 = time_events(torch, fn, steps)
+    import numpy as np
+    ok = spot_check(torch, t_in, lambda idx: t_col[idx, :].t().contiguous().cpu().numpy().view(np.uint32),
+                    [0, 1, cols // 3, cols - 1])
+    del t_col
+    return {
+        "workload": f"{side}x{side} square ({side} x {cols} u32) encoded into the column-major {cols} x {nkn} "
+                    "square, device-resident",
+        "ms_avg": round(avg, 4), "ms_p50": round(p50, 4),
+        "gbps": round(side * side * SHARE_BYTES * 8 / 1e9 / (avg / 1e3), 2),
+        "roofline": roofline_of(12 * side * cols, avg),
+        "spot_check": ok,
+    }
+
+
+def side_leg(zoda_hip, torch, side: int, seed: int, steps: int, e2e: bool):
+    """A second square size end to end: device-resident encode (HIP events,
+    roofline) and the pinned host-to-host encode."""
+    import numpy as np
+    cols = 128 * side
+    dev = torch.device("cuda:0")
+    t_in = torch.empty((side, cols), dtype=torch.int32, device=dev)
+    t_out = torch.empty((2 * side, cols), dtype=torch.int32, device=dev)
+    zoda_hip.fill_uniform(t_in, seed=seed)
+    sptr = torch.cuda.current_stream().cuda_stream
+    fn = lambda: zoda_hip.encode_device(t_in.data_ptr(), side, cols, side, t_out.data_ptr(), 0, sptr)
+    avg, p50 = time_events(torch, fn, steps)
+    pick = [0, 1, cols // 3, cols - 1]
+    ok = spot_check(torch, t_in, lambda idx: t_out[:, idx].contiguous().cpu().numpy().view(np.uint32), pick)
+    res = {
+        "workload": f"plain ZODA encode of a {side}x{side} square of 512 B shares ({side} x {cols} u32 -> "
+                    f"{2 * side} x {cols}), device-resident",
+        "ms_avg": round(avg, 4), "ms_p50": round(p50, 4), "steps": steps,
+        "gbps": round(side * side * SHARE_BYTES * 8 / 1e9 / (avg / 1e3), 2),
+        "roofline": roofline_of(12 * side * cols, avg, traffic_from_profile(side)),
+        "spot_check": ok,
+    }
+    if e2e:
+        pin_in = pin_out = None
+        try:
+            pin_in = zoda_hip.PinnedSquare(side * cols, 1)
+            pin_out = zoda_hip.PinnedSquare(2 * side * cols, 1)
+            a_in = pin_in.array.reshape(side, cols)
+            a_out = pin_out.array.reshape(2 * side, cols)
+            a_in[:] = t_in.cpu().numpy().view(np.uint32)
+            zoda_hip.encode_host(a_in, out=a_out)
+            times = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                zoda_hip.encode_host(a_in, out=a_out)
+                times.append(time.perf_counter() - t0)
+            times.sort()
+            ok_e2e = bool(np.array_equal(a_out[:, pick], t_out[:, pick].cpu().numpy().view(np.uint32)))
+            res["end_to_end_pinned"] = {
+                "gbps": round(side * side * SHARE_BYTES * 8 / 1e9 / times[1], 2),
+                "ms_p50": round(times[1] * 1e3, 2),
+                "window": "pinned row-major host in -> pinned host out, wall clock, median of 3",
+                "matches_device_encode": ok_e2e}
+        except Exception as ex:  # report, do not fail the line
+            res["end_to_end_pinned"] = {"error": str(ex)}
+        finally:
+            for pbuf in (pin_in, pin_out):
+                if pbuf is not None:
+                    pbuf.free()
+    del t_in, t_out
+    torch.cuda.empty_cache()
+    return res
+
+
 def _cpu_model():
+    """Model name, family/model/stepping, sockets and logical CPUs from /proc/cpuinfo."""
+    info = {}
+    sockets, logical = set(), 0
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" not in line:
+                    continue
+                key, val = (x.strip() for x in line.split(":", 1))
+                if key == "processor":
+                    logical += 1
+                elif key == "physical id":
+                    sockets.add(val)
+                elif key in ("model name", "cpu family", "model", "stepping", "cpu MHz", "cache size",
+                             "cpu cores") and key not in info:
+                    info[key] = val
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    return (f"{info.get('model name', 'unknown')} (family {info.get('cpu family', '?')}, model "
+            f"{info.get('model', '?')}, stepping {info.get('stepping', '?')}, {info.get('cache size', '?')} "
+            f"cache, {info.get('cpu cores', '?')} cores per socket, {len(sockets) or 1} socket(s), "
+            f"{logical} logical CPUs)")
 
 
 def main():
@@ -456,6 +664,14 @@ def main():
             e2e = {"error": str(ex)}
         result["end_to_end_pinned"] = e2e
 
+    if not args.no_col:
+        result["encode_col_device"] = column_major_leg(zoda_hip, torch, t_in, max(3, args.steps // 2))
+        torch.cuda.empty_cache()
+    if not args.no_dropin:
+        try:
+            result["gpu_xform_dropin"] = dropin_leg(zoda_hip, t_in, t_out)
+        except Exception as ex:  # report, do not fail the device-resident line
+            result["gpu_xform_dropin"] = {"error": str(ex)}
     if not args.no_rlc:
         result["rlc"] = rlc_leg(zoda_hip, torch, t_out)
     if not args.no_decode:
@@ -463,6 +679,8 @@ def main():
 
     del t_in, t_out
     torch.cuda.empty_cache()
+    if not args.no_4096 and side != 4096:
+        result["side_4096"] = side_leg(zoda_hip, torch, 4096, args.seed, 5, not args.no_e2e)
     if not args.no_batched:
         result["batched"] = batched_vs_plain(zoda_hip, torch, args.seed)
         torch.cuda.empty_cache()

@@ -147,15 +147,20 @@ int zoda_hip_set_variant(uint32_t log_side, int variant);
 /* ---- serving (no counterpart in the reference) ----------------------------- */
 
 typedef struct zoda_serve_stats {
-  double makespan_us;  /* first arrival to last completion */
-  uint32_t batches;    /* launches: batched launches (server) or requests (plain loop) */
-  uint32_t max_batch;  /* most requests in one launch */
-  uint64_t peak_bytes; /* largest HBM footprint admitted at once */
-  uint32_t rejected;   /* requests larger than the whole budget */
+  double makespan_us;       /* first arrival to last completion */
+  uint32_t batches;         /* launches: batched launches (server) or requests (plain loop) */
+  uint32_t max_batch;       /* most requests in one launch */
+  uint64_t peak_bytes;      /* largest footprint (inputs + outputs) admitted at once */
+  uint32_t rejected;        /* requests larger than the whole budget */
+  uint64_t pool_peak_bytes; /* most device memory the output pool held (<= budget) */
+  uint64_t pool_held_bytes; /* device memory the output pool holds now (in use + cached) */
 } zoda_serve_stats;
 
 /* Continuous-batching encode server on the current device.  Requests are
- * device-resident squares (zoda_square).  Each of `slots` launch slots
+ * device-resident squares (zoda_square); a request whose `out` is NULL gets
+ * its output from the server's device pool at admission (all pool memory,
+ * cached or in use, stays within hbm_budget; the request waits while its
+ * block does not fit).  Each of `slots` launch slots
  * (1..16) owns a stream.  Whenever one is free it takes the next launch: a
  * large request (side >= 256) at the head of the queue alone through the
  * fused single-square kernel (at most slots - 1 at once, in arrival order),
@@ -170,14 +175,23 @@ int zoda_server_create(uint64_t hbm_budget, uint32_t slots, zoda_server** server
 int zoda_server_submit(zoda_server* server, const zoda_square* request, uint64_t* ticket);
 /* state: 0 queued, 1 on the GPU, 2 done (latency_us = completion - arrival), 3 rejected */
 int zoda_server_poll(zoda_server* server, uint64_t ticket, int* state, double* latency_us);
+/* The pool output of a finished request (state 2) whose `out` was NULL: the
+ * device pointer stays valid until zoda_server_release returns it to the
+ * pool, which may hand it to the next request at once: the caller's reads of
+ * it must be complete (their stream synchronized) before the release.  A finished request is forgotten once it has been polled and holds
+ * no pool output; its ticket is then unknown (ZODA_EINVAL). */
+int zoda_server_output(zoda_server* server, uint64_t ticket, uint32_t** d_out);
+int zoda_server_release(zoda_server* server, uint64_t ticket);
 int zoda_server_wait_all(zoda_server* server);
 int zoda_server_stats(zoda_server* server, zoda_serve_stats* stats);
 int zoda_server_destroy(zoda_server* server);
 
 /* Replays a request trace in real time on this thread: request i arrives
  * arrival_us[i] microseconds after the start (non-decreasing).  mode 0 runs
- * the continuous-batching server above; mode 1 the plain loop, one
- * zoda_hip_encode_device per request in arrival order on one stream.
+ * the continuous-batching server above (outputs of requests with a NULL `out`
+ * come from its pool and go back at completion); mode 1 the plain loop, one
+ * zoda_hip_encode_device per request in arrival order on one stream
+ * (requests with a NULL `out` write to one block per output size).
  * latency_us[i] receives completion minus arrival (-1 if rejected). */
 int zoda_hip_serve_trace(const zoda_square* requests, const double* arrival_us, uint32_t count,
                          uint64_t hbm_budget, uint32_t slots, int mode, double* latency_us,

@@ -109,3 +109,77 @@ def test_online_server(zh, torch):
     assert e.value.code == -4
     small.close()
     srv.close()
+
+
+def pooled_trace(zh, torch, count, rate_per_s, seed, sides, own_every):
+    """A trace whose requests write to the server's output pool (out = 0),
+    except every `own_every`-th one, which gets a buffer of its own to check."""
+    chosen, arrivals, ins, outs, specs = trace(zh, torch, count, rate_per_s, seed, sides)
+    checked = list(range(0, count, own_every))
+    specs = [(di, do if i in checked else 0, k, c, n) for i, (di, do, k, c, n) in enumerate(specs)]
+    return chosen, arrivals, ins, outs, specs, checked
+
+
+def test_pool_bounded_by_budget(zh, torch):
+    sides = (64, 256, 512)
+    chosen, arrivals, ins, outs, specs, checked = pooled_trace(zh, torch, 30, 20000, 31, sides, 3)
+    out_max = max(2 * s * 128 * s * 4 for s in chosen)
+    budget = 3 * out_max                       # room for a few outputs at a time
+    print(f"[case] pooled trace budget={budget} sides={chosen}", flush=True)
+    lat, stats = zh.serve_trace(specs, arrivals, hbm_budget=budget, slots=3, mode=0)
+    print(f"[serve] pooled stats={stats}", flush=True)
+    assert (lat > 0).all() and stats["rejected"] == 0
+    assert 0 < stats["pool_peak_bytes"] <= budget
+    assert stats["peak_bytes"] <= budget
+    for i in checked:
+        assert torch.equal(outs[i], alone(zh, torch, ins[i])), (i, chosen[i])
+    # the plain loop takes the same trace (one output block per size)
+    lat1, stats1 = zh.serve_trace(specs, arrivals, hbm_budget=budget, slots=3, mode=1)
+    assert (lat1 > 0).all()
+    for i in checked:
+        assert torch.equal(outs[i], alone(zh, torch, ins[i])), (i, chosen[i])
+
+
+def test_online_pool_handoff(zh, torch):
+    import time
+    sides, _, ins, _, specs = trace(zh, torch, 12, 1000, seed=41, sides=(64, 128, 256))
+    out_max = max(2 * s * 128 * s * 4 for s in sides)
+    in_max = max(s * 128 * s * 4 for s in sides)
+    budget = 2 * out_max + in_max
+    srv = zh.Server(hbm_budget=budget, slots=2)
+    tickets = [srv.submit(di, 0, k, c, n) for (di, _, k, c, n) in specs]
+    pending = dict(zip(tickets, range(len(tickets))))
+    deadline = time.time() + 60
+    while pending and time.time() < deadline:
+        for t in list(pending):
+            state, lat = srv.poll(t)
+            if state != 2:
+                continue
+            i = pending.pop(t)
+            s = sides[i]
+            got = zh.device_view(srv.output(t), 2 * s, 128 * s).clone()
+            torch.cuda.synchronize()  # the copy has read the block before it goes back
+            srv.release(t)
+            assert lat > 0
+            assert torch.equal(got, alone(zh, torch, ins[i])), (i, s)
+    assert not pending, f"{len(pending)} requests not done"
+    stats = srv.stats()
+    assert 0 < stats["pool_peak_bytes"] <= budget
+    # a finished, polled and released request is forgotten
+    with pytest.raises(zh.ZodaError):
+        srv.poll(tickets[0])
+    srv.close()
+
+
+def test_unreleased_outputs_are_reported(zh, torch):
+    # Outputs held by finished requests that are never released block the
+    # queue once the pool is full: wait_all says so instead of waiting forever.
+    sides, _, ins, _, specs = trace(zh, torch, 4, 1000, seed=43, sides=(256,))
+    out_b = 2 * 256 * 128 * 256 * 4
+    srv = zh.Server(hbm_budget=out_b + 256 * 128 * 256 * 4, slots=1)
+    for (di, _, k, c, n) in specs:
+        srv.submit(di, 0, k, c, n)
+    with pytest.raises(zh.ZodaError) as e:
+        srv.wait_all()
+    assert e.value.code == -4
+    srv.close()

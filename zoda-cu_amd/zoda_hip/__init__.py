@@ -89,6 +89,8 @@ def _load():
         "zoda_server_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_double)]),
         "zoda_server_wait_all": (ctypes.c_int, [_vp]),
+        "zoda_server_output": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+        "zoda_server_release": (ctypes.c_int, [_vp, ctypes.c_uint64]),
         "zoda_server_stats": (ctypes.c_int, [_vp, _vp]),
         "zoda_server_destroy": (ctypes.c_int, [_vp]),
         "zoda_hip_serve_trace": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
@@ -377,7 +379,8 @@ def set_variant(side: int, variant: int) -> None:
 class ServeStats(ctypes.Structure):
     _fields_ = [("makespan_us", ctypes.c_double), ("batches", ctypes.c_uint32),
                 ("max_batch", ctypes.c_uint32), ("peak_bytes", ctypes.c_uint64),
-                ("rejected", ctypes.c_uint32)]
+                ("rejected", ctypes.c_uint32), ("pool_peak_bytes", ctypes.c_uint64),
+                ("pool_held_bytes", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -416,6 +419,18 @@ class Server:
                "zoda_server_poll")
         return state.value, lat.value
 
+    def output(self, ticket: int) -> int:
+        """Device pointer of a finished request's pool output (submitted with
+        d_out = 0); valid until release(ticket).  Reads of it must be complete
+        (stream synchronized) before the release: the block may go to the
+        next request at once."""
+        p = _vp()
+        _check(lib.zoda_server_output(self._srv, ticket, ctypes.byref(p)), "zoda_server_output")
+        return p.value
+
+    def release(self, ticket: int) -> None:
+        _check(lib.zoda_server_release(self._srv, ticket), "zoda_server_release")
+
     def wait_all(self) -> None:
         _check(lib.zoda_server_wait_all(self._srv), "zoda_server_wait_all")
 
@@ -434,6 +449,17 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def device_view(d_ptr: int, rows: int, cols: int):
+    """A torch int32 tensor over `rows x cols` u32 of device memory this
+    library handed out (e.g. Server.output); it does not own the memory."""
+    import torch
+
+    class _Dev:
+        __cuda_array_interface__ = {"shape": (rows, cols), "typestr": "<i4", "data": (d_ptr, False),
+                                    "version": 3, "strides": None}
+    return torch.as_tensor(_Dev(), device="cuda")
 
 
 # ---- random linear combinations (src/zoda/rlc.rs) ----
