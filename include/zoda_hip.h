@@ -220,6 +220,20 @@ int zoda_hip_rlc_coefficients(const char* data_root, uint32_t root_len, uint32_t
 int zoda_hip_rlc_rows(const uint32_t* d_square, uint32_t rows, uint32_t cols, const uint32_t* d_row_index,
                       uint32_t count, const uint32_t* d_coeffs, uint32_t* d_out, void* stream);
 
+/* zoda_hip_rlc_rows without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The partial sums
+ * live in a static 16 MiB device array instead of a per-call allocation:
+ * count * ceil(cols / 2048) must be <= 2^20 (e.g. 8192 rows of 262144
+ * columns), larger calls return ZODA_EINVAL.  Same arguments, results and
+ * stream semantics as zoda_hip_rlc_rows.  Every call on a device shares that
+ * device's array: calls that overlap on one GPU (concurrent streams, or a
+ * second call queued before the first has finished on another stream) race;
+ * calls ordered on one stream do not. */
+int zoda_hip_rlc_rows_noalloc(const uint32_t* d_square, uint32_t rows, uint32_t cols,
+                              const uint32_t* d_row_index, uint32_t count, const uint32_t* d_coeffs,
+                              uint32_t* d_out, void* stream);
+
 /* compute_data_root (utils.rs:51-59): SHA-256 of `count` host u32 values as
  * little-endian bytes in order, written as 64 lowercase hex digits and a NUL
  * to hex_out (65 bytes).  Runs on the host (one sequential hash chain). */

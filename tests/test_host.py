@@ -118,6 +118,17 @@ def test_kernel_emulations_under_sanitizers():
     assert "0 mismatches" in res.stdout
 
 
+def test_noalloc_rlc_rejects_sizes_beyond_its_scratch():
+    """The allocation-free RLC entry point refuses, before touching a device,
+    a call whose partial sums exceed its static scratch array."""
+    import zoda_hip
+    buf = np.zeros(16, dtype=np.uint32)
+    p = buf.ctypes.data
+    count = (1 << 20) + 1  # rows x column blocks just over the limit (one block)
+    rc = zoda_hip.lib.zoda_hip_rlc_rows_noalloc(p, count, 2048, None, count, p, p, None)
+    assert rc == -1 and "scratch" in zoda_hip.last_error()
+
+
 def test_uniform_generator_host_restatement():
     import zoda_hip
     a = zoda_hip.uniform_numpy(100000, 7)

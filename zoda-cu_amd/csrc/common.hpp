@@ -104,6 +104,8 @@ struct DeviceState {
   std::map<const void*, uint32_t> kernel_setup;
   // Twiddle tables of the slice-pipelined 2048 encode (encode_slice.hip).
   void* slice_tables = nullptr;
+  // This device's address of the RLC's static scratch array (rlc.hip).
+  uint32_t* rlc_scratch = nullptr;
 };
 
 // State of the calling thread's current device (created on first use).

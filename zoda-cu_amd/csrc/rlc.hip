@@ -233,6 +233,57 @@ __global__ void rlc_sum_kernel(const uint32_t* __restrict__ partial, uint32_t ro
   out[t] = mod_p(x);
 }
 
+// Partial sums of zoda_hip_rlc_rows_noalloc: a static device array, so the
+// entry point allocates nothing.  Each device holds its own copy for the
+// life of the process; host code reaches it only through the address
+// hipGetSymbolAddress gives for the current device (rlc_scratch_address).
+constexpr size_t RLC_SCRATCH_WORDS = size_t(1) << 22;  // 16 MiB: count * column blocks <= 2^20
+static __device__ __align__(256) uint32_t scratch[RLC_SCRATCH_WORDS];
+
+// Both kernels of a row RLC: the column blocks' sums into `partial` (every
+// entry the second kernel reads is written, nothing has to start at zero),
+// then their sum per row and limb.
+int launch_rlc_rows(const uint32_t* d_square, uint32_t rows, uint32_t cols, const uint32_t* d_row_index,
+                    uint32_t count, const uint32_t* d_coeffs, uint32_t* d_out, uint32_t* partial,
+                    uint32_t nblk, uint32_t nrb, hipStream_t s) {
+  hipLaunchKernelGGL(rlc_rows_kernel, dim3(nblk, nrb), dim3(RLC_NT), 0, s, d_square, rows, cols, d_row_index,
+                     count, reinterpret_cast<const uint4*>(d_coeffs), partial);
+  int rc = 0;
+  hip_note(rc, hipGetLastError(), "RLC kernel launch");
+  if (!rc) {
+    hipLaunchKernelGGL(rlc_sum_kernel, dim3((count * 4 + 255) / 256), dim3(256), 0, s, partial, count, nblk,
+                       d_out);
+    hip_note(rc, hipGetLastError(), "RLC reduction launch");
+  }
+  return rc;
+}
+
+// Argument checks shared by both row entry points; *nblk and *nrb get the grid.
+int check_rlc_rows(const uint32_t* d_square, uint32_t rows, uint32_t cols, const uint32_t* d_row_index,
+                   uint32_t count, const uint32_t* d_coeffs, uint32_t* d_out, uint32_t* nblk, uint32_t* nrb) {
+  if (!d_square || !d_coeffs || !d_out || rows == 0 || cols == 0 || count == 0)
+    return fail(ZODA_EINVAL, "bad arguments");
+  if (!d_row_index && count > rows) return fail(ZODA_EINVAL, "count %u exceeds rows %u", count, rows);
+  *nblk = (cols + RLC_COLS - 1) / RLC_COLS;
+  *nrb = (count + RLC_ROWS - 1) / RLC_ROWS;
+  if (*nrb > 65535) return fail(ZODA_EINVAL, "too many rows (%u)", count);
+  return 0;
+}
+
+// The current device's address of `scratch`, looked up once per device (with
+// that device current) and kept in its state.
+int rlc_scratch_address(DeviceState* st, uint32_t** out) {
+  std::lock_guard<std::recursive_mutex> lock(st->mu);
+  if (!st->rlc_scratch) {
+    void* p = nullptr;
+    ZODA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(ZODA_ESTATE, "no device address for the RLC scratch array");
+    st->rlc_scratch = static_cast<uint32_t*>(p);
+  }
+  *out = st->rlc_scratch;
+  return 0;
+}
+
 }  // namespace
 
 }  // namespace zoda
@@ -278,28 +329,35 @@ extern "C" int zoda_hip_rlc_rows(const uint32_t* d_square, uint32_t rows, uint32
                                  const uint32_t* d_row_index, uint32_t count, const uint32_t* d_coeffs,
                                  uint32_t* d_out, void* stream) {
   clear_error();
-  if (!d_square || !d_coeffs || !d_out || rows == 0 || cols == 0 || count == 0)
-    return fail(ZODA_EINVAL, "bad arguments");
-  if (!d_row_index && count > rows) return fail(ZODA_EINVAL, "count %u exceeds rows %u", count, rows);
+  uint32_t nblk = 0, nrb = 0;
+  ZODA_TRY(check_rlc_rows(d_square, rows, cols, d_row_index, count, d_coeffs, d_out, &nblk, &nrb));
   DeviceState* st = nullptr;
   ZODA_TRY(current_state(&st));
   hipStream_t s = (hipStream_t)stream;
-  const uint32_t nblk = (cols + RLC_COLS - 1) / RLC_COLS;
-  const uint32_t nrb = (count + RLC_ROWS - 1) / RLC_ROWS;
-  if (nrb > 65535) return fail(ZODA_EINVAL, "too many rows (%u)", count);
   uint32_t* partial = nullptr;
   ZODA_HIP(hipMallocAsync((void**)&partial, (size_t)count * nblk * 4 * sizeof(uint32_t), s));
-  hipLaunchKernelGGL(rlc_rows_kernel, dim3(nblk, nrb), dim3(RLC_NT), 0, s, d_square, rows, cols, d_row_index,
-                     count, reinterpret_cast<const uint4*>(d_coeffs), partial);
-  int rc = 0;
-  hip_note(rc, hipGetLastError(), "RLC kernel launch");
-  if (!rc) {
-    hipLaunchKernelGGL(rlc_sum_kernel, dim3((count * 4 + 255) / 256), dim3(256), 0, s, partial, count, nblk,
-                       d_out);
-    hip_note(rc, hipGetLastError(), "RLC reduction launch");
-  }
+  int rc = launch_rlc_rows(d_square, rows, cols, d_row_index, count, d_coeffs, d_out, partial, nblk, nrb, s);
   hip_note(rc, hipFreeAsync(partial, s), "hipFreeAsync");
   return rc;
+}
+
+extern "C" int zoda_hip_rlc_rows_noalloc(const uint32_t* d_square, uint32_t rows, uint32_t cols,
+                                         const uint32_t* d_row_index, uint32_t count, const uint32_t* d_coeffs,
+                                         uint32_t* d_out, void* stream) {
+  clear_error();
+  uint32_t nblk = 0, nrb = 0;
+  ZODA_TRY(check_rlc_rows(d_square, rows, cols, d_row_index, count, d_coeffs, d_out, &nblk, &nrb));
+  if ((size_t)count * nblk * 4 > RLC_SCRATCH_WORDS)
+    return fail(ZODA_EINVAL,
+                "%u rows x %u column blocks of %u columns need %zu scratch words, above the limit of %zu "
+                "(rows x blocks <= %zu)",
+                count, nblk, RLC_COLS, (size_t)count * nblk * 4, RLC_SCRATCH_WORDS, RLC_SCRATCH_WORDS / 4);
+  DeviceState* st = nullptr;
+  ZODA_TRY(current_state(&st));
+  uint32_t* partial = nullptr;
+  ZODA_TRY(rlc_scratch_address(st, &partial));
+  return launch_rlc_rows(d_square, rows, cols, d_row_index, count, d_coeffs, d_out, partial, nblk, nrb,
+                         (hipStream_t)stream);
 }
 
 // SHA-256 of the rows' values as little-endian u32 in row order, as lowercase

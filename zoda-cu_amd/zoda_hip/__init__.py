@@ -73,6 +73,8 @@ def _load():
         "zoda_hip_rlc_coefficients": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp]),
         "zoda_hip_rlc_rows": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp,
                                              _vp]),
+        "zoda_hip_rlc_rows_noalloc": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, ctypes.c_uint32,
+                                                     _vp, _vp, _vp]),
         "zoda_data_root": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_char_p]),
         "zoda_hip_ntt_mod": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                             ctypes.c_uint32, ctypes.c_int]),
