@@ -126,7 +126,6 @@ def batched_vs_plain(zoda_hip, torch, seed: int, reps: int = 7):
     import numpy as np
     rng = np.random.default_rng(seed)
     sides = rng.choice([32, 64, 128, 256, 512], size=96, p=[0.30, 0.30, 0.20, 0.15, 0.05]).tolist()
-    tuned = zoda_hip.autotune(sorted(set(sides)), iters=3)
     ins, outs, specs = [], [], []
     for i, sd in enumerate(sides):
         t_in = torch.empty((sd, 128 * sd), dtype=torch.int32, device="cuda")
@@ -137,6 +136,8 @@ def batched_vs_plain(zoda_hip, torch, seed: int, reps: int = 7):
         specs.append((t_in.data_ptr(), t_out.data_ptr(), sd, 128 * sd, sd))
     stream = torch.cuda.current_stream()
     sptr = stream.cuda_stream
+    torch.cuda.synchronize()
+    tuned = zoda_hip.autotune_batch(specs, iters=3)
 
     def plain():
         for (di, do, k, c, n) in specs:
@@ -178,9 +179,24 @@ def batched_vs_plain(zoda_hip, torch, seed: int, reps: int = 7):
         "plain_loop_gbps": round(payload * 8 / 1e9 / (t_plain / 1e3), 2),
         "batched_gbps": round(payload * 8 / 1e9 / (t_batch / 1e3), 2),
         "speedup": round(t_plain / t_batch, 3),
-        "autotuned_variant": {str(k): v["variant"] for k, v in tuned.items()},
+        "tuning_table": {
+            str(k): {
+                "choice": _candidate_name(v["candidates"][v["choice"]], v["choice"]),
+                "times_us": {_candidate_name(c, i): t for i, (c, t) in enumerate(zip(v["candidates"], v["times_us"]))},
+            } for k, v in tuned.items()},
+        "tuning_note": "per side of the batch (largest first): the four work-queue plans and the side's own "
+                       "fused launches, each timed as the whole batch (best of 3 after a warm-up, other sides at "
+                       "their current entries, starting from the untuned defaults); the fastest is kept in the "
+                       "device's table (zoda_hip_tune_save / _load persist it)",
         "bit_exact_vs_plain": ok,
     }
+
+
+def _candidate_name(c: dict, i: int) -> str:
+    d = "-".join(str(x) for x in c["digits"])
+    if c["kind"] == "direct":
+        return f"direct(W{c['W']},{c['threads']}t,{d})"
+    return f"q{i}(W{c['W']},{d},{c['twiddles']})"
 
 
 def served_vs_plain(zoda_hip, torch, seed: int, count: int, rates=(2000, 8000), slots: int = 3,

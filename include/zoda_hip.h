@@ -135,14 +135,35 @@ int zoda_hip_batch_plan_create(const zoda_square* squares, uint32_t count, zoda_
 int zoda_hip_batch_plan_launch(zoda_batch_plan* plan, void* stream);
 int zoda_hip_batch_plan_destroy(zoda_batch_plan* plan);
 
-/* Autotuner of the batched kernel: for each side (a power of two <= 4096),
- * times every plan variant (tile width, radix, stages per register pass) on a
- * synthetic side x 128*side square and keeps the fastest for this device.
- * times_us, if not NULL, receives count * 2 timings.  get/set read and
- * override the table entry of log2(side). */
+/* Autotuner of batched work: for each side (a power of two <= 4096) it times
+ * ZODA_TUNE_CANDIDATES candidates on a synthetic side x 128*side square --
+ * the ZODA_BATCH_VARIANTS plans of the persistent work-queue kernel (tile
+ * width, digit split, twiddles in LDS or global memory), and the side's own
+ * fused launch (ZODA_BATCH_DIRECT) -- and records the fastest in this
+ * device's table, which zoda_hip_encode_batch, batch plans and the server
+ * follow.  times_us, if not NULL, receives count * ZODA_TUNE_CANDIDATES
+ * microsecond timings (candidate order: queue variants 0..3, direct).
+ * get/set read and override the entry of log2(side) (0..3: the queue with
+ * that plan; ZODA_BATCH_DIRECT: own launches).  info[6] = {W, threads, R1,
+ * R2, RD, twiddles in LDS} of a candidate.  save/load persist the table as
+ * text, one line per side keyed by device name and CU count (load takes the
+ * lines of this device; *entries = how many). */
+#define ZODA_BATCH_VARIANTS 4
+#define ZODA_BATCH_DIRECT 4
+#define ZODA_TUNE_CANDIDATES 5
 int zoda_hip_autotune(const uint32_t* sides, uint32_t count, int iters, float* times_us);
+/* The same candidates tuned for one batch as it runs: for each side the
+ * batch holds (largest first) each candidate is timed as the whole batch with
+ * the other sides at their current entries, and the fastest kept (one sweep
+ * of coordinate descent from the current table).  Writes the squares' outputs
+ * as zoda_hip_encode_batch would.  times_us (optional): 13 *
+ * ZODA_TUNE_CANDIDATES floats, [log side][candidate], 0 for absent sides. */
+int zoda_hip_autotune_batch(const zoda_square* squares, uint32_t count, int iters, float* times_us);
 int zoda_hip_get_variant(uint32_t log_side, int* variant);
 int zoda_hip_set_variant(uint32_t log_side, int variant);
+int zoda_hip_batch_variant_info(uint32_t log_side, int variant, int* info);
+int zoda_hip_tune_save(const char* path);
+int zoda_hip_tune_load(const char* path, uint32_t* entries);
 
 /* ---- serving (no counterpart in the reference) ----------------------------- */
 

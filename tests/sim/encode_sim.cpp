@@ -177,9 +177,9 @@ struct SliceSim {
       if (front) slice::p2<PL, SP>(t, lds.data(), t2.data(), h(tid), k);
       if (back) slice::p1b<PL>(t, lds.data(), k);
       if (SP + 1 < PL::STEPS)
-        slice::stage_step<PL, !PL::EVEN_IN_MID>(t, b.obf, o, SP + 1, c4, l(tid), lds.data());
+        slice::stage_step<PL, !PL::even_in_mid(SP + 1)>(t, b.obf, o, SP + 1, c4, l(tid), lds.data());
       else
-        slice::stage_step<PL, !PL::EVEN_IN_MID>(t, b.obn, o, 0, c4, l(tid), lds.data());
+        slice::stage_step<PL, !PL::even_in_mid(0)>(t, b.obn, o, 0, c4, l(tid), lds.data());
       if (SP + 2 < PL::STEPS)
         slice::load_step<PL>(b.ibf, o, t.w, SP + 2, c4, l(tid));
       else
@@ -188,16 +188,16 @@ struct SliceSim {
     if constexpr (SP + 1 < PL::STEPS) step<SP + 1>(b, front, back);
   }
 
-  // the middle passes; with EVEN_IN_MID one step of even rows is copied per
-  // group, as on the device
+  // the middle passes; group KB copies step KB's even rows when its bit of
+  // EVEN_MID_MASK is set, as on the device
   template <int KB>
   void mid(int tid, const Bufs& b) {
     const slice::Tid t(tid);
     const slice::Offs<PL> o(t, cols);
     U4 ev[PL::QI];
-    if constexpr (PL::EVEN_IN_MID) slice::load_step<PL>(b.ibf, o, t.w, KB, cols * 4, ev);
+    if constexpr (PL::even_in_mid(KB)) slice::load_step<PL>(b.ibf, o, t.w, KB, cols * 4, ev);
     slice::mid_group<PL, KB>(h(tid), k);
-    if constexpr (PL::EVEN_IN_MID)
+    if constexpr (PL::even_in_mid(KB))
       for (int i = 0; i < PL::QI; i++) b.obf.store4(ev[i], o.ev, slice::Offs<PL>::s_out(KB, t.w, i, cols * 4));
     if constexpr (KB + 1 < PL::RB) mid<KB + 1>(tid, b);
   }
@@ -210,7 +210,7 @@ struct SliceSim {
       const slice::Tid t(tid);
       const slice::Offs<PL> o(t, cols);
       slice::load_step<PL>(gb(in, 0, true), o, t.w, 0, c4, l(tid));
-      slice::stage_step<PL, !PL::EVEN_IN_MID>(t, gb(out, 0, true), o, 0, c4, l(tid), lds.data());
+      slice::stage_step<PL, !PL::even_in_mid(0)>(t, gb(out, 0, true), o, 0, c4, l(tid), lds.data());
       slice::load_step<PL>(gb(in, 0, true), o, t.w, 1, c4, l(tid));
     }
     uint32_t tp = 0;
@@ -252,7 +252,7 @@ extern "C" int zoda_sim_encode_slice(const uint32_t* in, uint32_t k, uint32_t co
   return 0;
 }
 
-// variant < 0: the single-square plan of the side; 0/1: the batched plans.
+// variant < 0: the single-square plan of the side; 0..3: the batched plans.
 extern "C" int zoda_sim_encode_variant(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t n,
                                        uint32_t* out, int layout, int variant) {
   uint32_t nk = 1, L = 0;

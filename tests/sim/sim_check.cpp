@@ -56,7 +56,7 @@ void compare(const char* what, uint32_t k, uint32_t cols, int layout, int varian
     }
 }
 
-// The fused single-square plans (variant -1) and the batched plans (0, 1),
+// The fused single-square plans (variant -1) and the batched plans (0..3),
 // both layouts, ragged and whole-tile widths, systematic extension n = k.
 void check_plans() {
   const uint32_t widths[] = {1, 13, 64};
@@ -70,7 +70,7 @@ void check_plans() {
         for (int layout = 0; layout < 2; layout++) {
           std::vector<uint32_t> want((size_t)2 * k * cols), got(want.size());
           zo_encode(in.data(), k, cols, k, want.data(), layout, 1);
-          for (int variant = -1; variant <= 1; variant++) {
+          for (int variant = -1; variant <= 3; variant++) {
             std::fill(got.begin(), got.end(), 0xffffffffu);
             if (zoda_sim_encode_variant(in.data(), k, cols, k, got.data(), layout, variant) != 0) continue;
             compare("plan", k, cols, layout, variant, got, want);

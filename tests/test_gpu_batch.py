@@ -108,23 +108,62 @@ def test_graph_replay_one_kind(zh, torch, sides):
     plan.destroy()
 
 
-def test_autotune_variants_agree(zh, torch):
+def test_autotune_variants_agree(zh, torch, tmp_path):
     tuned = zh.autotune([32, 64, 128, 256, 512, 1024], iters=2)
-    print("[autotune]", tuned, flush=True)
+    print("[autotune]", {s: (e["choice"], e["times_us"]) for s, e in tuned.items()}, flush=True)
+    for s, e in tuned.items():
+        assert len(e["times_us"]) == zh.TUNE_CANDIDATES and all(0 < t < 1e29 for t in e["times_us"]), (s, e)
+        best = min(range(zh.TUNE_CANDIDATES), key=lambda v: e["times_us"][v])
+        assert e["times_us"][e["choice"]] == e["times_us"][best], (s, e)
+    # the table survives a save and load
+    path = str(tmp_path / "tune.txt")
+    zh.tune_save(path)
+    for s in tuned:
+        zh.set_variant(s, 0)
+    assert zh.tune_load(path) == 13
+    for s, e in tuned.items():
+        assert zh.get_variant(s) == e["choice"], s
+    # every candidate (four queue plans, own launches) gives the same squares
     sides = [16, 32, 64, 128, 256, 512, 1024]
     ins, outs, specs = make_batch(zh, torch, sides, seed=4, col_scale=4)
     results = []
-    for v in (0, 1):
+    for v in range(zh.TUNE_CANDIDATES):
+        print(f"[case] batch candidate {v}", flush=True)
         for s in sides:
             zh.set_variant(s, v)
+        for o in outs:
+            o.zero_()
         zh.encode_batch(specs)
         torch.cuda.synchronize()
         results.append([o.clone() for o in outs])
-    for s, a, b, t_in in zip(sides, results[0], results[1], ins):
-        assert torch.equal(a, b), s
-        assert torch.equal(a, plain(zh, torch, t_in, s)), s
+    for i, (s, t_in) in enumerate(zip(sides, ins)):
+        ref = plain(zh, torch, t_in, s)
+        for v in range(zh.TUNE_CANDIDATES):
+            assert torch.equal(results[v][i], ref), (s, v)
     for s, entry in tuned.items():
-        zh.set_variant(s, entry["variant"])
+        zh.set_variant(s, entry["choice"])
+
+
+def test_autotune_for_a_batch(zh, torch):
+    sides = [32, 64, 64, 128, 256, 512, 16]
+    ins, outs, specs = make_batch(zh, torch, sides, seed=8, col_scale=8)
+    saved = {s: zh.get_variant(s) for s in set(sides)}
+    tuned = zh.autotune_batch(specs, iters=2)
+    print("[autotune_batch]", {s: (e["choice"], e["times_us"]) for s, e in tuned.items()}, flush=True)
+    for s, e in tuned.items():
+        best = min(range(zh.TUNE_CANDIDATES), key=lambda v: e["times_us"][v])
+        assert e["times_us"][e["choice"]] == e["times_us"][best], (s, e)
+    # the tuner's own launches wrote the squares, and the tuned table encodes them the same
+    for t_in, t_out, s in zip(ins, outs, sides):
+        assert torch.equal(t_out, plain(zh, torch, t_in, s)), s
+    for o in outs:
+        o.zero_()
+    zh.encode_batch(specs)
+    torch.cuda.synchronize()
+    for t_in, t_out, s in zip(ins, outs, sides):
+        assert torch.equal(t_out, plain(zh, torch, t_in, s)), s
+    for s, v in saved.items():
+        zh.set_variant(s, v)
 
 
 def test_batch_argument_errors(zh, torch):

@@ -195,10 +195,10 @@ def test_slice_kernel_emulation_extreme_values():
         assert np.array_equal(sim_encode_slice(data), O.encode(data, threads=4))
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("k", [1, 3, 8, 16, 17, 32, 64, 100, 128, 256, 512, 1024, 2048, 4096])
 def test_batched_plans_emulation(k, variant):
-    # the plans of the persistent batched kernel (both autotuner variants)
+    # the plans of the persistent batched kernel (every autotuner variant)
     rng = np.random.default_rng(10 * k + variant)
     cols = 70 if k <= 1024 else 12
     data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)

@@ -8,8 +8,11 @@
 // kernel, so [reset, persistent kernel] can be captured once into a hipGraph
 // and replayed (zoda_hip_batch_plan_*).  The plan variant of each side comes
 // from the per-device autotune table (zoda_hip_autotune).
+#include <stdio.h>
+
 #include <algorithm>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "batch.hpp"
@@ -19,11 +22,14 @@
 namespace zoda {
 
 // Two instantiations of each batched kernel: one for every side, and one for
-// the sides the queue holds when the large squares run as their own launches
-// (below 2^DIRECT_MIN_LOG).  A kernel's register budget is its largest
+// queues of small squares only (below 2^DIRECT_MIN_LOG, as when the large
+// squares run as their own launches).  A kernel's register budget is its largest
 // plan's, so the small one fits several times more work-groups on a CU.
 constexpr int BATCH_SMALL_LOG = DIRECT_MIN_LOG - 1;
-constexpr int BATCH_BRANCHES = 2;  // side branches of the batch's direct (large-square) launches
+constexpr int BATCH_BRANCHES = 2;
+static_assert(ZODA_BATCH_VARIANTS == BATCH_VARIANTS && ZODA_BATCH_DIRECT == BATCH_DIRECT &&
+                  ZODA_TUNE_CANDIDATES == BATCH_VARIANTS + 1,
+              "the header's tuning constants match the plans");  // side branches of the batch's direct (large-square) launches
 
 template <class PL, bool FULL>
 __device__ __forceinline__ void batch_tile_body(int tid, uint32_t tile, const SquareDesc& d, int32_t* lds,
@@ -288,8 +294,9 @@ static int launch_batch_direct(zoda_batch_plan* p, hipStream_t stream) {
 
 // Builds descriptors, the largest-first item list and the per-(side, variant)
 // table pointers on the device; with `graph`, also captures the launch.  With
-// allow_direct, squares of side >= 2^DIRECT_MIN_LOG run as their own fused
-// launches beside the queue (the autotuner times the queue alone).
+// allow_direct, squares of the sides the device's table marks direct run as
+// their own fused launches beside the queue (the autotuner times the queue
+// plans alone).
 static int build_plan(const zoda_square* squares, uint32_t count, bool graph, bool allow_direct,
                       zoda_batch_plan** out) {
   clear_error();
@@ -303,7 +310,7 @@ static int build_plan(const zoda_square* squares, uint32_t count, bool graph, bo
 
   std::vector<zoda_square> direct, queued;
   for (uint32_t i = 0; i < count; i++)
-    (allow_direct && log2_ceil(squares[i].k) >= DIRECT_MIN_LOG ? direct : queued).push_back(squares[i]);
+    (allow_direct && st->batch_direct[log2_ceil(squares[i].k)] ? direct : queued).push_back(squares[i]);
   std::stable_sort(direct.begin(), direct.end(),
                    [](const zoda_square& a, const zoda_square& b) { return a.k > b.k; });
   std::vector<SquareDesc> descs(queued.size());
@@ -421,42 +428,87 @@ extern "C" int zoda_hip_get_variant(uint32_t log_side, int* variant) {
   if (!variant || (int)log_side > BATCH_MAX_LOG) return fail(ZODA_EINVAL, "bad arguments");
   DeviceState* st = nullptr;
   ZODA_TRY(current_state(&st));
-  *variant = st->batch_variant[log_side];
+  std::lock_guard<std::recursive_mutex> lock(st->mu);
+  *variant = st->batch_direct[log_side] ? BATCH_DIRECT : st->batch_variant[log_side];
   return 0;
 }
 
 extern "C" int zoda_hip_set_variant(uint32_t log_side, int variant) {
   clear_error();
-  if ((int)log_side > BATCH_MAX_LOG || variant < 0 || variant >= BATCH_VARIANTS)
+  if ((int)log_side > BATCH_MAX_LOG || variant < 0 || variant > BATCH_DIRECT)
     return fail(ZODA_EINVAL, "bad arguments");
   DeviceState* st = nullptr;
   ZODA_TRY(current_state(&st));
   std::lock_guard<std::recursive_mutex> lock(st->mu);
-  st->batch_variant[log_side] = variant;
+  st->batch_direct[log_side] = variant == BATCH_DIRECT;
+  if (variant < BATCH_VARIANTS) st->batch_variant[log_side] = variant;
   return 0;
 }
 
-// For each listed side, times every batched-plan variant on one synthetic
-// square of that side (side rows x 128*side columns, device-resident, the best
-// of `iters` launches of a one-square batch) and keeps the fastest in the
-// device's table.  times_us (optional) receives count * BATCH_VARIANTS values.
+extern "C" int zoda_hip_batch_variant_info(uint32_t log_side, int variant, int* info) {
+  clear_error();
+  if (!info || (int)log_side > BATCH_MAX_LOG || variant < 0 || variant > BATCH_DIRECT)
+    return fail(ZODA_EINVAL, "bad arguments");
+  auto put = [&](auto plan) -> int {
+    using PL = decltype(plan);
+    info[0] = PL::W;
+    info[1] = PL::NT;
+    info[2] = PL::R1;
+    info[3] = PL::R2;
+    info[4] = PL::RD;
+    info[5] = PL::TWL ? 1 : 0;
+    return 0;
+  };
+  if (variant == BATCH_DIRECT) {
+    if (with_encode_plan((int)log_side, put) != 0) return fail(ZODA_EINVAL, "no plan for 2^%u", log_side);
+    if (log_side == 11) {  // the slice-pipelined kernel (32-column tiles, 512 threads, LDS twiddles)
+      info[0] = 32;
+      info[1] = 512;
+      info[2] = 4;
+      info[3] = 3;
+      info[4] = 4;
+    }
+  } else {
+    with_batch_plan((int)log_side, variant, put);
+  }
+  return 0;
+}
+
+// For each listed side, times every candidate on one synthetic square of that
+// side (side rows x 128*side columns, device-resident, the best of `iters`
+// launches after one warm-up): the BATCH_VARIANTS work-queue plans, as a
+// one-square batch, and the side's own fused launch.  The fastest queue plan
+// becomes the side's variant, and the side runs direct when its own launch
+// beat it.  times_us (optional) receives count * ZODA_TUNE_CANDIDATES values.
 extern "C" int zoda_hip_autotune(const uint32_t* sides, uint32_t count, int iters, float* times_us) {
   clear_error();
   if (!sides || count == 0 || iters < 1) return fail(ZODA_EINVAL, "bad arguments");
+  for (uint32_t i = 0; i < count; i++)
+    if (!is_pow2(sides[i]) || (int)log2_ceil(sides[i]) > BATCH_MAX_LOG)
+      return fail(ZODA_EINVAL, "side %u must be a power of two <= 2^%d", sides[i], BATCH_MAX_LOG);
   DeviceState* st = nullptr;
   ZODA_TRY(current_state(&st));
   hipStream_t s;
   ZODA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  hipEvent_t e0, e1;
-  ZODA_HIP(hipEventCreate(&e0));
-  ZODA_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = 0;
+  hip_note(rc, hipEventCreate(&e0), "autotune event");
+  if (!rc) hip_note(rc, hipEventCreate(&e1), "autotune event");
+  // best of `iters` timed runs of fn after one warm-up, in microseconds
+  auto time_it = [&](auto&& fn, float* best) {
+    *best = 1e30f;
+    for (int it = 0; it <= iters && !rc; it++) {
+      hip_note(rc, hipEventRecord(e0, s), "autotune event");
+      if (!rc) rc = fn();
+      if (!rc) hip_note(rc, hipEventRecord(e1, s), "autotune event");
+      if (!rc) hip_note(rc, hipEventSynchronize(e1), "autotune launch");
+      float ms = 0;
+      if (!rc) hip_note(rc, hipEventElapsedTime(&ms, e0, e1), "autotune timing");
+      if (!rc && it > 0) *best = std::min(*best, ms * 1000.0f);
+    }
+  };
   for (uint32_t i = 0; i < count && !rc; i++) {
     const uint32_t side = sides[i];
-    if (!is_pow2(side) || (int)log2_ceil(side) > BATCH_MAX_LOG) {
-      rc = fail(ZODA_EINVAL, "side %u must be a power of two <= 2^%d", side, BATCH_MAX_LOG);
-      break;
-    }
     const uint32_t L = log2_ceil(side);
     const uint64_t cols = 128ull * side;
     uint32_t *din = nullptr, *dout = nullptr;
@@ -468,6 +520,7 @@ extern "C" int zoda_hip_autotune(const uint32_t* sides, uint32_t count, int iter
       break;
     }
     rc = launch_fill_uniform(din, side * cols, side, s);
+    const int keep_variant = st->batch_variant[L];
     float best = 1e30f;
     int best_v = 0;
     for (int v = 0; v < BATCH_VARIANTS && !rc; v++) {
@@ -476,29 +529,173 @@ extern "C" int zoda_hip_autotune(const uint32_t* sides, uint32_t count, int iter
       zoda_batch_plan* p = nullptr;
       rc = build_plan(&sq, 1, false, false, &p);
       if (rc) break;
-      float vbest = 1e30f;
-      for (int it = 0; it <= iters && !rc; it++) {
-        hipEventRecord(e0, s);
-        rc = launch_batch_direct(p, s);
-        hipEventRecord(e1, s);
-        hip_note(rc, hipEventSynchronize(e1), "autotune launch");
-        float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
-        if (it > 0) vbest = std::min(vbest, ms * 1000.0f);  // first launch warms up
-      }
+      float t = 0;
+      time_it([&] { return launch_batch_direct(p, s); }, &t);
       free_plan(p);
-      if (times_us) times_us[i * BATCH_VARIANTS + v] = vbest;
-      if (vbest < best) {
-        best = vbest;
+      if (times_us) times_us[i * ZODA_TUNE_CANDIDATES + v] = t;
+      if (t < best) {
+        best = t;
         best_v = v;
       }
     }
-    st->batch_variant[L] = best_v;
+    float t_direct = 1e30f;
+    if (!rc) {
+      time_it([&] { return launch_encode(st, din, side, (uint32_t)cols, side, dout, ZODA_LAYOUT_ROW, s); },
+              &t_direct);
+      if (times_us) times_us[i * ZODA_TUNE_CANDIDATES + BATCH_DIRECT] = t_direct;
+    }
+    {
+      std::lock_guard<std::recursive_mutex> lock(st->mu);
+      st->batch_variant[L] = rc ? keep_variant : best_v;
+      if (!rc) st->batch_direct[L] = t_direct < best;
+    }
     (void)hipFree(din);
     (void)hipFree(dout);
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   (void)hipStreamDestroy(s);
   return rc;
+}
+
+// Tunes the device's table for one batch as it will run: for each side the
+// batch holds (largest first), every candidate of that side is timed as the
+// whole batch (best of `iters` launches after a warm-up), with the other
+// sides at their current entries, and the fastest is kept (one sweep of
+// coordinate descent from the current table, so the result is never slower
+// than the entries it started from, as measured).  The squares' outputs are
+// written as by zoda_hip_encode_batch.  times_us (optional, 13 *
+// ZODA_TUNE_CANDIDATES floats) receives [log side][candidate] timings, 0 for
+// sides the batch lacks.
+extern "C" int zoda_hip_autotune_batch(const zoda_square* squares, uint32_t count, int iters, float* times_us) {
+  clear_error();
+  if (!squares || count == 0 || iters < 1) return fail(ZODA_EINVAL, "bad arguments");
+  for (uint32_t i = 0; i < count; i++) ZODA_TRY(check_batch_square(squares[i], i));
+  DeviceState* st = nullptr;
+  ZODA_TRY(current_state(&st));
+  if (times_us)
+    for (int i = 0; i < (BATCH_MAX_LOG + 1) * ZODA_TUNE_CANDIDATES; i++) times_us[i] = 0.0f;
+  bool present[BATCH_MAX_LOG + 1] = {};
+  for (uint32_t i = 0; i < count; i++) present[log2_ceil(squares[i].k)] = true;
+  hipStream_t s;
+  ZODA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  hip_note(rc, hipEventCreate(&e0), "autotune event");
+  if (!rc) hip_note(rc, hipEventCreate(&e1), "autotune event");
+  auto set_choice = [&](int L, int v) {
+    std::lock_guard<std::recursive_mutex> lock(st->mu);
+    st->batch_direct[L] = v == BATCH_DIRECT;
+    if (v < BATCH_VARIANTS) st->batch_variant[L] = v;
+  };
+  for (int L = BATCH_MAX_LOG; L >= BATCH_MIN_LOG && !rc; L--) {
+    if (!present[L]) continue;
+    const int start_v = st->batch_direct[L] ? BATCH_DIRECT : st->batch_variant[L];
+    const int start_q = st->batch_variant[L];
+    float best = 1e30f;
+    int best_v = start_v;
+    for (int v = 0; v < ZODA_TUNE_CANDIDATES && !rc; v++) {
+      set_choice(L, v);
+      if (v == BATCH_DIRECT) {  // the queue plan the side keeps while it runs direct
+        std::lock_guard<std::recursive_mutex> lock(st->mu);
+        st->batch_variant[L] = start_q;
+      }
+      zoda_batch_plan* p = nullptr;
+      rc = build_plan(squares, count, false, true, &p);
+      if (rc) break;
+      float t = 1e30f;
+      for (int it = 0; it <= iters && !rc; it++) {
+        hip_note(rc, hipEventRecord(e0, s), "autotune event");
+        if (!rc) rc = launch_batch_direct(p, s);
+        if (!rc) hip_note(rc, hipEventRecord(e1, s), "autotune event");
+        if (!rc) hip_note(rc, hipEventSynchronize(e1), "autotune launch");
+        float ms = 0;
+        if (!rc) hip_note(rc, hipEventElapsedTime(&ms, e0, e1), "autotune timing");
+        if (!rc && it > 0) t = std::min(t, ms * 1000.0f);
+      }
+      free_plan(p);
+      if (times_us) times_us[L * ZODA_TUNE_CANDIDATES + v] = t;
+      if (t < best) {
+        best = t;
+        best_v = v;
+      }
+    }
+    // the queue plan stays the best queue candidate's (when the side runs direct too)
+    int best_q = start_q;
+    if (times_us) {
+      float bq = 1e30f;
+      for (int v = 0; v < BATCH_VARIANTS; v++)
+        if (times_us[L * ZODA_TUNE_CANDIDATES + v] < bq) {
+          bq = times_us[L * ZODA_TUNE_CANDIDATES + v];
+          best_q = v;
+        }
+    }
+    set_choice(L, rc ? start_v : best_v);
+    if (!rc && best_v == BATCH_DIRECT && times_us) {
+      std::lock_guard<std::recursive_mutex> lock(st->mu);
+      st->batch_variant[L] = best_q;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
+// The device's name and CU count, the key of a tuning table on disk.
+static int device_key(DeviceState* st, std::string* key) {
+  hipDeviceProp_t prop;
+  ZODA_HIP(hipGetDeviceProperties(&prop, st->device));
+  char buf[300];
+  snprintf(buf, sizeof buf, "%s/%dcu", prop.gcnArchName, prop.multiProcessorCount);
+  for (char* c = buf; *c; c++)
+    if (*c == ' ') *c = '_';
+  *key = buf;
+  return 0;
+}
+
+// Tuning table persistence: one line per log side, "<device> <L> <variant>"
+// (variant ZODA_BATCH_DIRECT for the side's own fused launch).  Saving
+// replaces the file; loading takes the lines of this device's name and CU
+// count and skips the others.
+extern "C" int zoda_hip_tune_save(const char* path) {
+  clear_error();
+  if (!path) return fail(ZODA_EINVAL, "bad arguments");
+  DeviceState* st = nullptr;
+  ZODA_TRY(current_state(&st));
+  std::string key;
+  ZODA_TRY(device_key(st, &key));
+  FILE* f = fopen(path, "w");
+  if (!f) return fail(ZODA_EINVAL, "cannot write %s", path);
+  {
+    std::lock_guard<std::recursive_mutex> lock(st->mu);
+    for (int L = BATCH_MIN_LOG; L <= BATCH_MAX_LOG; L++)
+      fprintf(f, "%s %d %d\n", key.c_str(), L, st->batch_direct[L] ? BATCH_DIRECT : st->batch_variant[L]);
+  }
+  if (fclose(f) != 0) return fail(ZODA_EINVAL, "writing %s failed", path);
+  return 0;
+}
+
+extern "C" int zoda_hip_tune_load(const char* path, uint32_t* entries) {
+  clear_error();
+  if (!path) return fail(ZODA_EINVAL, "bad arguments");
+  DeviceState* st = nullptr;
+  ZODA_TRY(current_state(&st));
+  std::string key;
+  ZODA_TRY(device_key(st, &key));
+  FILE* f = fopen(path, "r");
+  if (!f) return fail(ZODA_EINVAL, "cannot read %s", path);
+  char dev[300];
+  int L = 0, v = 0;
+  uint32_t n = 0;
+  std::lock_guard<std::recursive_mutex> lock(st->mu);
+  while (fscanf(f, "%299s %d %d", dev, &L, &v) == 3) {
+    if (key != dev || L < BATCH_MIN_LOG || L > BATCH_MAX_LOG || v < 0 || v > BATCH_DIRECT) continue;
+    st->batch_direct[L] = v == BATCH_DIRECT;
+    if (v < BATCH_VARIANTS) st->batch_variant[L] = v;
+    n++;
+  }
+  fclose(f);
+  if (entries) *entries = n;
+  return 0;
 }

@@ -90,7 +90,11 @@ struct DeviceState {
   std::map<uint32_t, MontTables> mont;           // standard roots, keyed by log n
   std::map<uint32_t, EncodeTableSet> encode;     // keyed by encode_table_key()
   std::map<uint32_t, DecodeTableSet> decode;     // keyed by log n
-  int batch_variant[16] = {0};                   // autotuned batched-plan variant per log side
+  // Autotuned batched plan per log side: the work-queue plan variant, and
+  // whether the side's squares run as their own fused launches instead.
+  int batch_variant[16] = {0};
+  bool batch_direct[16] = {false, false, false, false, false, false, false, false,
+                           true,  true,  true,  true,  true,  true,  true,  true};
   std::map<uint32_t, uint32_t> batch_grid;       // batched kernels' persistent grid by LDS bytes
   // Drop-in entry points: tables built from the caller's roots.
   std::map<uint32_t, std::vector<uint32_t>> user_roots;
@@ -146,9 +150,10 @@ bool slice_fits(uint32_t k, uint32_t cols, int layout);
 int launch_slice(DeviceState* st, const uint32_t* d_in, uint32_t cols, uint32_t* d_out,
                  hipStream_t stream, bool dry);
 
-// Squares of side >= 2^DIRECT_MIN_LOG fill the device with their own fused
-// kernel: batched and served work runs them as separate launches next to the
-// persistent work queue of the smaller squares.
+// Before tuning, squares of side >= 2^DIRECT_MIN_LOG run as their own fused
+// launches in batched and served work (their kernel fills the device), next
+// to the persistent work queue of the smaller squares; the autotuner decides
+// per side (DeviceState::batch_direct).
 constexpr uint32_t DIRECT_MIN_LOG = 8;
 
 // Seeded uniform generator (api.hip).

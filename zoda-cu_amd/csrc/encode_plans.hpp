@@ -53,15 +53,18 @@ int with_encode_plan(int L, F&& f) {
 namespace zoda {
 
 // Plans of the batched persistent kernel: one block size (256) for every
-// side, at most 80 KiB of LDS so two work-groups share a CU, and two variants
-// per side (tile width W, digit split = radix and stages per register pass)
-// for the autotuner to choose from.  Up to side 1024 the twiddle tables sit
-// in LDS behind the tile (loaded when a work-group changes plan); the two
-// largest sides read them from global memory.
+// side, at most 80 KiB of LDS so two work-groups share a CU, and four
+// variants per side for the autotuner to choose from (besides running the
+// side's squares as their own fused launches): tile width W, digit split
+// (radix and stages per register pass) and twiddle tables in LDS behind the
+// tile (loaded when a work-group changes plan) or read from global memory.
+// Variant 0 is the default before tuning.  The two largest sides have one
+// plan each (their tiles leave no room for variants under the LDS limit).
 constexpr int BATCH_NT = 256;
 constexpr int BATCH_MIN_LOG = 0;
 constexpr int BATCH_MAX_LOG = 12;
-constexpr int BATCH_VARIANTS = 2;
+constexpr int BATCH_VARIANTS = 4;
+constexpr int BATCH_DIRECT = BATCH_VARIANTS;  // tuning choice: the side's own fused launch
 // 80 KiB less 64 bytes for the kernel's static LDS (its work-item slot).
 constexpr int BATCH_LDS_WORDS = 20480 - 16;
 
@@ -72,54 +75,70 @@ using B2a = EncodePlan<2, 256, BATCH_NT, 0, 0, 2, true>;
 using B3a = EncodePlan<3, 256, BATCH_NT, 0, 0, 3, true>;
 using B4a = EncodePlan<4, 256, BATCH_NT, 0, 0, 4, true>;
 using B4b = EncodePlan<4, 64, BATCH_NT, 2, 0, 2, true>;
-using B5b = EncodePlan<5, 64, BATCH_NT, 3, 0, 2, true>;
+using B4c = EncodePlan<4, 128, BATCH_NT, 2, 0, 2, true>;
+using B4d = EncodePlan<4, 64, BATCH_NT, 2, 0, 2, false>;
+using B5a = EncodePlan<5, 64, BATCH_NT, 3, 0, 2, true>;
+using B5b = EncodePlan<5, 128, BATCH_NT, 3, 0, 2, true>;
+using B5c = EncodePlan<5, 64, BATCH_NT, 2, 0, 3, true>;
+using B5d = EncodePlan<5, 64, BATCH_NT, 3, 0, 2, false>;
 using B6a = EncodePlan<6, 64, BATCH_NT, 3, 0, 3, true>;
 using B6b = EncodePlan<6, 32, BATCH_NT, 3, 0, 3, true>;
+using B6c = EncodePlan<6, 128, BATCH_NT, 3, 0, 3, true>;
+using B6d = EncodePlan<6, 64, BATCH_NT, 3, 0, 3, false>;
 using B7a = EncodePlan<7, 64, BATCH_NT, 4, 0, 3, true>;
 using B7b = EncodePlan<7, 32, BATCH_NT, 4, 0, 3, true>;
+using B7c = EncodePlan<7, 64, BATCH_NT, 3, 0, 4, true>;
+using B7d = EncodePlan<7, 64, BATCH_NT, 4, 0, 3, false>;
 using B8a = EncodePlan<8, 64, BATCH_NT, 4, 0, 4, true>;
 using B8b = EncodePlan<8, 32, BATCH_NT, 3, 2, 3, true>;
+using B8c = EncodePlan<8, 32, BATCH_NT, 4, 0, 4, true>;
+using B8d = EncodePlan<8, 64, BATCH_NT, 4, 0, 4, false>;
 using B9a = EncodePlan<9, 32, BATCH_NT, 3, 3, 3, true>;
 using B9b = EncodePlan<9, 32, BATCH_NT, 5, 0, 4, true>;
+using B9c = EncodePlan<9, 16, BATCH_NT, 3, 3, 3, true>;
+using B9d = EncodePlan<9, 32, BATCH_NT, 3, 3, 3, false>;
 using B10a = EncodePlan<10, 16, BATCH_NT, 4, 3, 3, true>;
 using B10b = EncodePlan<10, 16, BATCH_NT, 5, 0, 5, true>;
+using B10c = EncodePlan<10, 8, BATCH_NT, 4, 3, 3, true>;
+using B10d = EncodePlan<10, 16, BATCH_NT, 4, 3, 3, false>;
 using B11a = EncodePlan<11, 8, BATCH_NT, 4, 4, 3>;
 using B12a = EncodePlan<12, 4, BATCH_NT, 4, 4, 4>;
 
-// Calls f(Plan{}) for batched plan (L, variant); variants a side lacks fall
-// back to variant 0.  Returns -1 for L out of range.
+// Calls f(Plan{}) for batched plan (L, variant); sides with fewer plans map
+// the other variants onto theirs.  Returns -1 for L out of range.
 template <class PL>
 constexpr bool fits_batch() {
   return PL::NT == BATCH_NT && PL::LDS_WORDS <= BATCH_LDS_WORDS;
 }
-static_assert(fits_batch<B8a>() && fits_batch<B9a>() && fits_batch<B9b>() && fits_batch<B10a>() &&
-                  fits_batch<B10b>() && fits_batch<B11a>() && fits_batch<B12a>(),
-              "batched plans must fit the batched kernel's LDS");
 
 template <class F>
 ZHD int with_batch_plan(int L, int variant, F&& f) {
-  switch (L * BATCH_VARIANTS + (variant & 1)) {
-    case 0: case 1: return f(B0a{});
-    case 2: case 3: return f(B1a{});
-    case 4: case 5: return f(B2a{});
-    case 6: case 7: return f(B3a{});
-    case 8: return f(B4a{});
-    case 9: return f(B4b{});
-    case 10: case 11: return f(B5b{});
-    case 12: return f(B6a{});
-    case 13: return f(B6b{});
-    case 14: return f(B7a{});
-    case 15: return f(B7b{});
-    case 16: return f(B8a{});
-    case 17: return f(B8b{});
-    case 18: return f(B9a{});
-    case 19: return f(B9b{});
-    case 20: return f(B10a{});
-    case 21: return f(B10b{});
-    case 22: case 23: return f(B11a{});
-    case 24: case 25: return f(B12a{});
+  const int v = variant & (BATCH_VARIANTS - 1);
+  switch (L) {
+    case 0: return f(B0a{});
+    case 1: return f(B1a{});
+    case 2: return f(B2a{});
+    case 3: return f(B3a{});
+    case 4: return v == 0 ? f(B4a{}) : v == 1 ? f(B4b{}) : v == 2 ? f(B4c{}) : f(B4d{});
+    case 5: return v == 0 ? f(B5a{}) : v == 1 ? f(B5b{}) : v == 2 ? f(B5c{}) : f(B5d{});
+    case 6: return v == 0 ? f(B6a{}) : v == 1 ? f(B6b{}) : v == 2 ? f(B6c{}) : f(B6d{});
+    case 7: return v == 0 ? f(B7a{}) : v == 1 ? f(B7b{}) : v == 2 ? f(B7c{}) : f(B7d{});
+    case 8: return v == 0 ? f(B8a{}) : v == 1 ? f(B8b{}) : v == 2 ? f(B8c{}) : f(B8d{});
+    case 9: return v == 0 ? f(B9a{}) : v == 1 ? f(B9b{}) : v == 2 ? f(B9c{}) : f(B9d{});
+    case 10: return v == 0 ? f(B10a{}) : v == 1 ? f(B10b{}) : v == 2 ? f(B10c{}) : f(B10d{});
+    case 11: return f(B11a{});
+    case 12: return f(B12a{});
     default: return -1;
   }
 }
+
+template <class... P>
+constexpr bool all_fit_batch() {
+  return (fits_batch<P>() && ...);
+}
+static_assert(all_fit_batch<B0a, B1a, B2a, B3a, B4a, B4b, B4c, B4d, B5a, B5b, B5c, B5d, B6a, B6b, B6c, B6d, B7a,
+                            B7b, B7c, B7d, B8a, B8b, B8c, B8d, B9a, B9b, B9c, B9d, B10a, B10b, B10c, B10d, B11a,
+                            B12a>(),
+              "batched plans must fit the batched kernel's LDS");
 
 }  // namespace zoda

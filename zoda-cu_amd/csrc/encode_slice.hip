@@ -64,7 +64,7 @@ struct Bufs {
 // instead of after it.  The two waves that share a SIMD (w and w + 4) take
 // opposite orders, so that while one of them waits for the address unit to
 // take its loads and stores, the other keeps the SIMD busy.
-template <bool EVM, int SP, bool MF>
+template <unsigned EVM, int SP, bool MF>
 __device__ __forceinline__ void step(const Ctx& c, uint32_t c4, bool front, bool back, const Bufs& b,
                                      U4 (&ld)[PL::QI], int32_t (&held)[PL::RB][PL::RC]) {
   // odd rows of step SP - 1 (of the previous iteration's last step for SP = 0)
@@ -75,9 +75,9 @@ __device__ __forceinline__ void step(const Ctx& c, uint32_t c4, bool front, bool
   // then the loads of step SP + 2
   auto stage = [&] {
     if constexpr (SP + 1 < PL::STEPS)
-      slice::stage_step<PL, !EVM>(c.t, b.obf, c.o, SP + 1, c4, ld, c.lds);
+      slice::stage_step<PL, !((EVM >> (SP + 1)) & 1u)>(c.t, b.obf, c.o, SP + 1, c4, ld, c.lds);
     else
-      slice::stage_step<PL, !EVM>(c.t, b.obn, c.o, 0, c4, ld, c.lds);
+      slice::stage_step<PL, !(EVM & 1u)>(c.t, b.obn, c.o, 0, c4, ld, c.lds);
     if constexpr (SP + 2 < PL::STEPS)
       slice::load_step<PL>(b.ibf, c.o, c.wu, SP + 2, c4, ld);
     else
@@ -108,15 +108,17 @@ __device__ __forceinline__ void step(const Ctx& c, uint32_t c4, bool front, bool
   if constexpr (SP + 1 < PL::STEPS) step<EVM, SP + 1, MF>(c, c4, front, back, b, ld, held);
 }
 
-// The middle passes of the front tile; with EVM its even rows are copied
-// (read again, 16 bytes a lane) between them, so that the CU keeps memory
-// requests in flight through the passes that have no memory work of their own.
-template <bool EVM, int KB>
+// The middle passes of the front tile; group KB copies step KB's even rows
+// (read again, 16 bytes a lane) when bit KB of EVM is set, so that the CU
+// keeps memory requests in flight through the passes that have no memory
+// work of their own.
+template <unsigned EVM, int KB>
 __device__ __forceinline__ void mid(const Ctx& c, uint32_t c4, const Bufs& b, int32_t (&held)[PL::RB][PL::RC]) {
   U4 ev[PL::QI];
-  if constexpr (EVM) slice::load_step<PL>(b.ibf, c.o, c.wu, KB, c4, ev);
+  constexpr bool COPY = (EVM >> KB) & 1u;
+  if constexpr (COPY) slice::load_step<PL>(b.ibf, c.o, c.wu, KB, c4, ev);
   slice::mid_group<PL, KB>(held, c.k);
-  if constexpr (EVM) {
+  if constexpr (COPY) {
     group_fence();
 #pragma unroll
     for (int i = 0; i < PL::QI; i++) b.obf.store4(ev[i], c.o.ev, slice::Offs<PL>::s_out(KB, c.wu, i, c4));
@@ -131,7 +133,7 @@ constexpr bool kSplitOrder = true;
 // prologue, the pipelined tiles and the epilogue.  Both orders are complete
 // functions, so no register state crosses the branch between them (at such a
 // join the two orders' register assignments would have to meet, and spill).
-template <bool EVM, bool MF>
+template <unsigned EVM, bool MF>
 __device__ __forceinline__ void run_tiles(const uint32_t* in, uint32_t* out, uint32_t cols,
                                           uint32_t ntiles, const Ctx& c, int tid, int32_t* lds) {
   const uint32_t b = blockIdx.x, g = gridDim.x;
@@ -150,7 +152,7 @@ __device__ __forceinline__ void run_tiles(const uint32_t* in, uint32_t* out, uin
     const uint32_t t0 = live ? tile(0) : 0u;
     const GBuf ib(in + (size_t)t0 * PL::W, live), ob(out + (size_t)t0 * PL::W, live);
     slice::load_step<PL>(ib, c.o, c.wu, 0, cols * 4u, ld);
-    slice::stage_step<PL, !EVM>(c.t, ob, c.o, 0, cols * 4u, ld, lds);
+    slice::stage_step<PL, !(EVM & 1u)>(c.t, ob, c.o, 0, cols * 4u, ld, lds);
     slice::load_step<PL>(ib, c.o, c.wu, 1, cols * 4u, ld);
   }
   lds_barrier();  // the LDS tables and the first IN
@@ -181,7 +183,7 @@ __device__ __forceinline__ void run_tiles(const uint32_t* in, uint32_t* out, uin
   slice::store_odd<PL>(c.t, ob, c.o, PL::STEPS - 1, cols * 4u, lds);
 }
 
-template <bool EVM>
+template <unsigned EVM>
 __global__ __launch_bounds__(PL::NT) void encode_slice(const uint32_t* __restrict__ in,
                                                        uint32_t* __restrict__ out, uint32_t cols,
                                                        uint32_t ntiles, const Tw2* __restrict__ t1,
@@ -237,7 +239,7 @@ int launch_slice(DeviceState* st, const uint32_t* d_in, uint32_t cols, uint32_t*
   const Tw2* t1 = nullptr;
   const double *t2 = nullptr, *t3 = nullptr;
   ZODA_TRY(get_slice_tables(st, &t1, &t2, &t3));
-  const void* fn = (const void*)encode_slice<PL::EVEN_IN_MID>;
+  const void* fn = (const void*)encode_slice<PL::EVEN_MID_MASK>;
   uint32_t grid = 0;
   ZODA_TRY(kernel_setup(st, fn, PL::NT, PL::LDS_BYTES, true, &grid));
   if (dry) return 0;

@@ -3,8 +3,9 @@
 // Requests (device-resident squares, as zoda_square) queue in arrival order.
 // Each of `slots` launch slots owns a stream, a completion event and a work
 // counter.  Whenever a slot is free it takes the next launch:
-//   * a large request (side >= 2^DIRECT_MIN_LOG: its own fused kernel fills
-//     the device) at the head of the queue goes alone through the single-square
+//   * a large request (a side the device's tuning table runs direct: its own
+//     fused kernel fills the device) at the head of the queue goes alone
+//     through the single-square
 //     kernel; at most slots - 1 of them run at once, in arrival order; while
 //     the head is a large request that has to wait (for a slot or memory),
 //     nothing passes it;
@@ -191,7 +192,10 @@ class Server {
     r.sq = sq;
     r.arrival_us = arrival_us;
     r.bytes = request_bytes(sq);
-    r.big = log2_ceil(sq.k) >= DIRECT_MIN_LOG;
+    {
+      std::lock_guard<std::recursive_mutex> dl(st_->mu);
+      r.big = st_->batch_direct[log2_ceil(sq.k)];
+    }
     r.pooled = sq.out == nullptr;
     r.release_on_done = r.pooled && release_on_done;
     if (!r.big) {  // resolve the batched plan and its tables now, off the launch path

@@ -247,12 +247,16 @@ struct Plan2048 {
   // < QI covers the rows of slice w / 4 with nA (or j3) = (4 w + i) mod 16
   static constexpr int QI = 4;
   static_assert(QI * 8 * WAVES == SPS * RA * RB, "one 16-byte access per lane covers a step");
-  // Even output rows: copied (read again) between the middle passes (true),
-  // or written from the staged input rows in the steps (false).  The copy
-  // keeps memory requests in flight through the middle passes, which have no
-  // memory work of their own.
-  static constexpr bool EVEN_IN_MID = true;
-  static_assert(!EVEN_IN_MID || RB == STEPS, "one step of even rows per middle group");
+  // Even output rows of step s: copied (read again) in middle group s when
+  // bit s of EVEN_MID_MASK is set, else written from the staged input rows in
+  // the steps.  The copies keep memory requests in flight through the middle
+  // passes, which have no memory work of their own; with every other step's
+  // even rows there, memory work and arithmetic are in about the same ratio
+  // in the middle passes as in the steps (measured best of none, every
+  // fourth, every other and all steps).
+  static constexpr unsigned EVEN_MID_MASK = 0x55u;
+  static constexpr ZHD bool even_in_mid(int s) { return (EVEN_MID_MASK >> s) & 1u; }
+  static_assert(EVEN_MID_MASK == 0 || RB == STEPS, "one step of even rows per middle group");
   static constexpr int XWORDS = SPS * RB * RA * W;  // one LDS buffer (int32 / u32)
   static constexpr int T1_PAIRS = RB * RA;          // [nB][kA] (w, w/p) pairs, global memory
   static constexpr int T2_WORDS = RC * RA * RB;     // [c][kA][kB] doubles in LDS

@@ -86,6 +86,10 @@ def _load():
         "zoda_hip_autotune": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_int, _vp]),
         "zoda_hip_get_variant": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(ctypes.c_int)]),
         "zoda_hip_set_variant": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int]),
+        "zoda_hip_batch_variant_info": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int, _vp]),
+        "zoda_hip_tune_save": (ctypes.c_int, [ctypes.c_char_p]),
+        "zoda_hip_autotune_batch": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_int, _vp]),
+        "zoda_hip_tune_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32)]),
         "zoda_server_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_vp)]),
         "zoda_server_submit": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_uint64)]),
         "zoda_server_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int),
@@ -352,18 +356,66 @@ class BatchPlan:
             pass
 
 
+BATCH_VARIANTS = 4
+BATCH_DIRECT = 4          # tuning choice: the side's own fused launch
+TUNE_CANDIDATES = BATCH_VARIANTS + 1
+
+
+def candidate_info(side: int, variant: int) -> dict:
+    """Tile width, threads, digit split and twiddle placement of a tuning
+    candidate (variant 0..3: a work-queue plan; BATCH_DIRECT: the side's own
+    fused kernel)."""
+    info = (ctypes.c_int * 6)()
+    _check(lib.zoda_hip_batch_variant_info(side.bit_length() - 1, variant, info), "zoda_hip_batch_variant_info")
+    return {"kind": "direct" if variant == BATCH_DIRECT else "queue", "W": info[0], "threads": info[1],
+            "digits": [info[2], info[3], info[4]], "twiddles": "lds" if info[5] else "global"}
+
+
 def autotune(sides, iters: int = 3) -> dict:
-    """Times every batched-plan variant per side and keeps the fastest;
-    returns {side: {"variant": v, "times_us": [...]}}."""
+    """Times every candidate per side (the four work-queue plans and the
+    side's own fused launch) and records the fastest in the device's table;
+    returns {side: {"choice": c, "times_us": [5 floats], "candidates": [...]}}
+    (choice 0..3: queue plan, BATCH_DIRECT: own launch)."""
     sides = list(sides)
     arr = (ctypes.c_uint32 * len(sides))(*sides)
-    times = (ctypes.c_float * (2 * len(sides)))()
+    times = (ctypes.c_float * (TUNE_CANDIDATES * len(sides)))()
     _check(lib.zoda_hip_autotune(ctypes.addressof(arr), len(sides), iters, ctypes.addressof(times)),
            "zoda_hip_autotune")
     out = {}
     for i, side in enumerate(sides):
-        out[side] = {"variant": get_variant(side), "times_us": [times[2 * i], times[2 * i + 1]]}
+        out[side] = {"choice": get_variant(side),
+                     "times_us": [round(times[TUNE_CANDIDATES * i + v], 2) for v in range(TUNE_CANDIDATES)],
+                     "candidates": [candidate_info(side, v) for v in range(TUNE_CANDIDATES)]}
     return out
+
+
+def autotune_batch(specs, iters: int = 3) -> dict:
+    """Tunes the device's table for this batch as it runs (coordinate descent
+    over its sides, largest first; see zoda_hip_autotune_batch); returns
+    {side: {"choice": c, "times_us": [5 floats], "candidates": [...]}} for the
+    sides the batch holds."""
+    arr = _squares(specs)
+    times = (ctypes.c_float * (13 * TUNE_CANDIDATES))()
+    _check(lib.zoda_hip_autotune_batch(ctypes.addressof(arr), len(specs), iters, ctypes.addressof(times)),
+           "zoda_hip_autotune_batch")
+    out = {}
+    for side in sorted({int(sp[2]) for sp in specs}):
+        side = 1 << (side - 1).bit_length()
+        L = side.bit_length() - 1
+        out[side] = {"choice": get_variant(side),
+                     "times_us": [round(times[TUNE_CANDIDATES * L + v], 2) for v in range(TUNE_CANDIDATES)],
+                     "candidates": [candidate_info(side, v) for v in range(TUNE_CANDIDATES)]}
+    return out
+
+
+def tune_save(path: str) -> None:
+    _check(lib.zoda_hip_tune_save(path.encode()), "zoda_hip_tune_save")
+
+
+def tune_load(path: str) -> int:
+    n = ctypes.c_uint32(0)
+    _check(lib.zoda_hip_tune_load(path.encode(), ctypes.byref(n)), "zoda_hip_tune_load")
+    return n.value
 
 
 def get_variant(side: int) -> int:
