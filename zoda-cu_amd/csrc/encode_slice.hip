@@ -30,6 +30,17 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// A wave raises its issue priority while it hands its loads and stores to the
+// address unit, so that the SIMD's arbiter lets it through ahead of the other
+// wave's arithmetic and its memory work starts early (1 % over equal
+// priorities).
+__device__ __forceinline__ void mem_prio(bool high) {
+  if (high)
+    __builtin_amdgcn_s_setprio(3);
+  else
+    __builtin_amdgcn_s_setprio(0);
+}
+
 struct Ctx {
   slice::Tid t;
   slice::Offs<PL> o;
@@ -84,25 +95,33 @@ __device__ __forceinline__ void step(const Ctx& c, uint32_t c4, bool front, bool
       slice::load_step<PL>(b.ibn, c.o, c.wu, SP + 2 - PL::STEPS, c4, ld);
   };
   if constexpr (MF) {
+    mem_prio(true);
     odd();
     group_fence();
+    mem_prio(false);
   }
   if (front) slice::p1<PL>(c.t, c.lds, c.t1w, c.k);
   if (back) slice::p2b<PL, SP>(c.t, held, c.lds, c.t3, c.k);
   if constexpr (!MF) {
     group_fence();
+    mem_prio(true);
     odd();
+    mem_prio(false);
   }
   lds_barrier();
   if constexpr (MF) {
+    mem_prio(true);
     stage();
     group_fence();
+    mem_prio(false);
   }
   if (front) slice::p2<PL, SP>(c.t, c.lds, c.t2, held, c.k);
   if (back) slice::p1b<PL>(c.t, c.lds, c.k);
   if constexpr (!MF) {
     group_fence();
+    mem_prio(true);
     stage();
+    mem_prio(false);
   }
   lds_barrier();
   if constexpr (SP + 1 < PL::STEPS) step<EVM, SP + 1, MF>(c, c4, front, back, b, ld, held);
