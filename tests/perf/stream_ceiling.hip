@@ -79,17 +79,17 @@ __global__ __launch_bounds__(NT) void stream_dword(const uint32_t* __restrict__ 
   if (MODE == RO && acc == 0x12345678u) sink[threadIdx.x] = acc;
 }
 
-// 16-byte shape of a 32-column tile: lane q = t % 8 covers columns 4q..4q+3,
-// thread t covers row t / 8 + k NT / 8.
-template <int NT, int MODE, bool NTS>
+// 16-byte shape of a W-column tile: lane q = t % (W / 4) covers columns
+// 4q..4q+3, thread t covers row t / (W / 4) + k NT / (W / 4).
+template <int W, int NT, int MODE, bool NTS>
 __global__ __launch_bounds__(NT) void stream_x4(const v4u* __restrict__ in, v4u* __restrict__ out,
                                                 uint32_t* sink) {
-  constexpr uint32_t W = 32, RPS = NT / 8, CQ = C / 4;
+  constexpr uint32_t QW = W / 4, RPS = NT / QW, CQ = C / 4;
   static_assert(S % (4 * RPS) == 0, "whole row batches");
-  const uint32_t ntiles = C / W, q = threadIdx.x % 8, r0 = threadIdx.x / 8;
+  const uint32_t ntiles = C / W, q = threadIdx.x % QW, r0 = threadIdx.x / QW;
   uint32_t acc = 0;
   for (uint32_t l = blockIdx.x; l < ntiles; l += gridDim.x) {
-    const uint32_t c = xcd_tile(l, ntiles) * (W / 4) + q;
+    const uint32_t c = xcd_tile(l, ntiles) * QW + q;
     for (uint32_t r = r0; r < S; r += 4 * RPS) {
       v4u v[4];
 #pragma unroll
@@ -147,11 +147,11 @@ void dword_line(const uint32_t* in, uint32_t* out, uint32_t* sink, int cus, int 
   report("dword", W, NT, per_cu, MODE, NTS, ms);
 }
 
-template <int NT, int MODE, bool NTS>
+template <int W, int NT, int MODE, bool NTS>
 void x4_line(const uint32_t* in, uint32_t* out, uint32_t* sink, int cus, int per_cu) {
-  const float ms = time_kernel(stream_x4<NT, MODE, NTS>, cus * per_cu, NT, reinterpret_cast<const v4u*>(in),
+  const float ms = time_kernel(stream_x4<W, NT, MODE, NTS>, cus * per_cu, NT, reinterpret_cast<const v4u*>(in),
                                reinterpret_cast<v4u*>(out), sink);
-  report("x4", 32, NT, per_cu, MODE, NTS, ms);
+  report("x4", W, NT, per_cu, MODE, NTS, ms);
 }
 
 int main() {
@@ -193,19 +193,23 @@ int main() {
   dword_line<64, 512, RW, false>(in, out, sink, cus, 2);
   dword_line<64, 512, RW, true>(in, out, sink, cus, 2);
   dword_line<16, 1024, RW, true>(in, out, sink, cus, 1);
-  x4_line<512, RW, false>(in, out, sink, cus, 1);
-  x4_line<512, RW, true>(in, out, sink, cus, 1);
-  x4_line<512, RW, true>(in, out, sink, cus, 2);
+  x4_line<32, 512, RW, false>(in, out, sink, cus, 1);
+  x4_line<32, 512, RW, true>(in, out, sink, cus, 1);
+  x4_line<32, 512, RW, true>(in, out, sink, cus, 2);
+  x4_line<16, 512, RW, false>(in, out, sink, cus, 1);
+  x4_line<16, 512, RW, true>(in, out, sink, cus, 1);
+  x4_line<8, 512, RW, false>(in, out, sink, cus, 1);
+  x4_line<64, 512, RW, true>(in, out, sink, cus, 1);
   dword_line<8, 512, WO, true>(in, out, sink, cus, 2);
   dword_line<16, 512, WO, true>(in, out, sink, cus, 2);
   dword_line<32, 512, WO, true>(in, out, sink, cus, 2);
   dword_line<64, 512, WO, true>(in, out, sink, cus, 2);
-  x4_line<512, WO, true>(in, out, sink, cus, 1);
+  x4_line<32, 512, WO, true>(in, out, sink, cus, 1);
   dword_line<8, 512, RO, false>(in, out, sink, cus, 2);
   dword_line<16, 512, RO, false>(in, out, sink, cus, 2);
   dword_line<32, 512, RO, false>(in, out, sink, cus, 2);
   dword_line<64, 512, RO, false>(in, out, sink, cus, 2);
-  x4_line<512, RO, false>(in, out, sink, cus, 1);
+  x4_line<32, 512, RO, false>(in, out, sink, cus, 1);
   CK(hipFree(in));
   CK(hipFree(out));
   CK(hipFree(sink));

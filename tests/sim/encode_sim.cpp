@@ -8,6 +8,7 @@
 // product library does not contain or call this code.
 #include <stdint.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "../../zoda-cu_amd/csrc/encode_plans.hpp"
@@ -141,38 +142,40 @@ extern "C" int zoda_sim_encode_persistent(const uint32_t* in, uint32_t k, uint32
 // passes run between.  16-byte accesses, zero-record buffers for inactive
 // parts, and the loads one step ahead in per-thread registers, as on the GPU.
 namespace {
-using SP2048 = slice::Plan2048;
+template <class PL>
 struct SliceSim {
-  using PL = SP2048;
   using Held = int32_t[PL::RB][PL::RC];
+  using B = std::conditional_t<PL::WIDE, slice::TileRef, GBuf>;
+  using Tid = slice::Tid<PL>;
   const uint32_t* in;
   uint32_t* out;
   uint32_t cols;
-  std::vector<slice::Tw2> t1;
-  std::vector<double> t2, t3;
+  std::vector<typename PL::T1T> t1;
+  std::vector<typename PL::TwT> t2, t3;
   std::vector<int32_t> lds;
   std::vector<std::array<int32_t, PL::RB * PL::RC>> held;
   std::vector<std::array<U4, PL::QI>> ld;
   slice::MulK k;
   Held& h(int tid) { return *reinterpret_cast<Held*>(held[tid].data()); }
   U4 (&l(int tid))[PL::QI] { return *reinterpret_cast<U4(*)[PL::QI]>(ld[tid].data()); }
+  const typename PL::T1T* t1w(int w) const { return t1.data() + w * (PL::T1_PAIRS / PL::RB); }
 
   struct Bufs {
-    GBuf ibf, obf, ibn, obn, obb, obp;
+    B ibf, obf, ibn, obn, obb, obp;
   };
 
   template <int SP>
   void step(const Bufs& b, bool front, bool back) {
     const uint32_t c4 = cols * 4;
     for (int tid = 0; tid < PL::NT; tid++) {
-      const slice::Tid t(tid);
+      const Tid t(tid);
       const slice::Offs<PL> o(t, cols);
       slice::store_odd<PL>(t, SP == 0 ? b.obp : b.obb, o, SP == 0 ? PL::STEPS - 1 : SP - 1, c4, lds.data());
-      if (front) slice::p1<PL>(t, lds.data(), t1.data() + t.w * PL::RA, k);
+      if (front) slice::p1<PL>(t, lds.data(), t1w(t.w), k);
       if (back) slice::p2b<PL, SP>(t, h(tid), lds.data(), t3.data(), k);
     }
     for (int tid = 0; tid < PL::NT; tid++) {
-      const slice::Tid t(tid);
+      const Tid t(tid);
       const slice::Offs<PL> o(t, cols);
       if (front) slice::p2<PL, SP>(t, lds.data(), t2.data(), h(tid), k);
       if (back) slice::p1b<PL>(t, lds.data(), k);
@@ -192,22 +195,26 @@ struct SliceSim {
   // EVEN_MID_MASK is set, as on the device
   template <int KB>
   void mid(int tid, const Bufs& b) {
-    const slice::Tid t(tid);
+    const Tid t(tid);
     const slice::Offs<PL> o(t, cols);
     U4 ev[PL::QI];
     if constexpr (PL::even_in_mid(KB)) slice::load_step<PL>(b.ibf, o, t.w, KB, cols * 4, ev);
     slice::mid_group<PL, KB>(h(tid), k);
-    if constexpr (PL::even_in_mid(KB))
-      for (int i = 0; i < PL::QI; i++) b.obf.store4(ev[i], o.ev, slice::Offs<PL>::s_out(KB, t.w, i, cols * 4));
+    if constexpr (PL::even_in_mid(KB)) slice::store_even<PL>(b.obf, o, t.w, KB, cols * 4, ev);
     if constexpr (KB + 1 < PL::RB) mid<KB + 1>(tid, b);
   }
 
-  GBuf gb(const uint32_t* base, uint32_t tile, bool live) const { return GBuf(base + (size_t)tile * PL::W, live); }
+  B gb(const uint32_t* base, uint32_t tile, bool live) const {
+    if constexpr (PL::WIDE)
+      return slice::TileRef{base + (size_t)tile * PL::W, live};
+    else
+      return GBuf(base + (size_t)tile * PL::W, live);
+  }
 
   void run() {
     const uint32_t ntiles = cols / PL::W, c4 = cols * 4;
     for (int tid = 0; tid < PL::NT; tid++) {
-      const slice::Tid t(tid);
+      const Tid t(tid);
       const slice::Offs<PL> o(t, cols);
       slice::load_step<PL>(gb(in, 0, true), o, t.w, 0, c4, l(tid));
       slice::stage_step<PL, !PL::even_in_mid(0)>(t, gb(out, 0, true), o, 0, c4, l(tid), lds.data());
@@ -225,18 +232,17 @@ struct SliceSim {
       tp = tb;
     }
     for (int tid = 0; tid < PL::NT; tid++) {
-      const slice::Tid t(tid);
+      const Tid t(tid);
       slice::store_odd<PL>(t, gb(out, tp, true), slice::Offs<PL>(t, cols), PL::STEPS - 1, c4, lds.data());
     }
   }
 };
-}  // namespace
 
-extern "C" int zoda_sim_encode_slice(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out) {
-  using PL = SP2048;
-  if (k != (uint32_t)PL::S || cols == 0 || cols % PL::W != 0) return -3;
-  if (((uint64_t)(2 * PL::S - 1) * cols + PL::W) * 4 >= (1ull << 32)) return -3;
-  SliceSim sim;
+template <class PL>
+int sim_slice(const uint32_t* in, uint32_t cols, uint32_t* out) {
+  if (cols == 0 || cols % PL::W != 0) return -3;
+  if (!PL::WIDE && ((uint64_t)(2 * PL::S - 1) * cols + PL::W) * 4 >= (1ull << 32)) return -3;
+  SliceSim<PL> sim;
   sim.in = in;
   sim.out = out;
   sim.cols = cols;
@@ -250,6 +256,14 @@ extern "C" int zoda_sim_encode_slice(const uint32_t* in, uint32_t k, uint32_t co
   sim.k = slice::make_mulk();
   sim.run();
   return 0;
+}
+}  // namespace
+
+// The slice kernel of side k (2048 or 4096) run on the host.
+extern "C" int zoda_sim_encode_slice(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out) {
+  if (k == (uint32_t)slice::Plan2048::S) return sim_slice<slice::Plan2048>(in, cols, out);
+  if (k == (uint32_t)slice::Plan4096::S) return sim_slice<slice::Plan4096>(in, cols, out);
+  return -3;
 }
 
 // variant < 0: the single-square plan of the side; 0..3: the batched plans.

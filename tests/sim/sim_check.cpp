@@ -107,9 +107,9 @@ void check_persistent_and_slice() {
     zo_encode(in.data(), k, cols, k, want.data(), 0, 1);
     compare("persistent", k, cols, 0, -1, got, want);
   }
-  for (uint32_t cols : {32u, 96u}) {
+  for (uint32_t kc : {2048u * 1000 + 32, 2048u * 1000 + 96, 4096u * 1000 + 16, 4096u * 1000 + 48}) {
+    const uint32_t k = kc / 1000, cols = kc % 1000;
     for (int extreme = 0; extreme < 2; extreme++) {
-      const uint32_t k = 2048;
       const std::vector<uint32_t> in = square(k, cols, extreme);
       std::vector<uint32_t> want((size_t)2 * k * cols), got(want.size());
       zo_encode(in.data(), k, cols, k, want.data(), 0, 1);

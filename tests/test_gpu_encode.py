@@ -160,6 +160,18 @@ def test_slice_kernel_tile_counts(zh, torch, tiles):
     assert np.array_equal(gpu_encode(zh, torch, data), O.encode(data, threads=16))
 
 
+@pytest.mark.parametrize("tiles", [1, 7, 255, 256, 257, 600])
+def test_slice_kernel_tile_counts_4096(zh, torch, tiles):
+    # side 4096 in the row layout with whole 16-column tiles runs the
+    # slice-pipelined kernel's 4096 plan (radix-32 passes split between two
+    # threads, a buffer base per step); every value against the oracle
+    side, cols = 4096, 16 * tiles
+    rng = np.random.default_rng(tiles + 4096)
+    data = rng.integers(0, P, size=(side, cols), dtype=np.uint32)
+    print_case("slice-tiles", side=side, cols=cols)
+    assert np.array_equal(gpu_encode(zh, torch, data), O.encode(data, threads=16))
+
+
 def test_column_major_slab_2048(zh, torch):
     # the exact slab zoda_hip_encode_host encodes at 2048^2 (4096 columns of
     # 2048 rows, whole tiles, column-major output), every value checked

@@ -177,22 +177,28 @@ def test_kernel_emulation_full_tiles(k, cols):
     assert np.array_equal(sim_encode(data, layout=1).T, expect)
 
 
-@pytest.mark.parametrize("cols", [32, 64, 96])
-def test_slice_kernel_emulation(cols):
-    # the slice-pipelined side-2048 kernel's per-thread code and order of work
-    # (front of tile i beside the back of tile i - 1, held registers handed
-    # over slot by slot, input and odd rows staged in LDS), run on the host
+@pytest.mark.parametrize("k,cols", [(2048, 32), (2048, 64), (2048, 96), (4096, 16), (4096, 32), (4096, 80)])
+def test_slice_kernel_emulation(k, cols):
+    # the slice-pipelined kernel's per-thread code and order of work (front of
+    # tile i beside the back of tile i - 1, held registers handed over slot by
+    # slot, input and odd rows staged in LDS; at side 4096 the radix-32 passes
+    # split between two threads and per-step buffer bases), run on the host
     from helpers import sim_encode_slice
-    rng = np.random.default_rng(cols)
-    data = rng.integers(0, P, size=(2048, cols), dtype=np.uint32)
+    rng = np.random.default_rng(cols + k)
+    data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)
     assert np.array_equal(sim_encode_slice(data), O.encode(data, threads=4))
 
 
-def test_slice_kernel_emulation_extreme_values():
+@pytest.mark.parametrize("k,cols", [(2048, 32), (4096, 16)])
+def test_slice_kernel_emulation_extreme_values(k, cols):
     from helpers import sim_encode_slice
     for fill in (0, 1, P - 1):
-        data = np.full((2048, 32), fill, dtype=np.uint32)
+        data = np.full((k, cols), fill, dtype=np.uint32)
         assert np.array_equal(sim_encode_slice(data), O.encode(data, threads=4))
+    data = np.zeros((k, cols), dtype=np.uint32)
+    data[::3] = P - 1
+    data[1::3] = P - 2
+    assert np.array_equal(sim_encode_slice(data), O.encode(data, threads=4))
 
 
 @pytest.mark.parametrize("variant", [0, 1, 2, 3])

@@ -107,7 +107,7 @@ struct DeviceState {
   // limit is raised, no grid needed).
   std::map<const void*, uint32_t> kernel_setup;
   // Twiddle tables of the slice-pipelined 2048 encode (encode_slice.hip).
-  void* slice_tables = nullptr;
+  void* slice_tables[2] = {nullptr, nullptr};  // sides 2048, 4096
   // This device's address of the RLC's static scratch array (rlc.hip).
   uint32_t* rlc_scratch = nullptr;
 };
@@ -144,10 +144,11 @@ int launch_encode(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t co
 int kernel_setup(DeviceState* st, const void* fn, int nt, size_t lds_bytes, bool persistent,
                  uint32_t* grid);
 
-// The slice-pipelined encode of side 2048 (whole 32-column tiles, row layout,
-// output below 4 GiB from each tile's base): true when it takes the shape.
+// The slice-pipelined encode of sides 2048 and 4096 (whole 32- resp.
+// 16-column tiles, row layout, offsets within its addressing limits): true
+// when it takes the shape.
 bool slice_fits(uint32_t k, uint32_t cols, int layout);
-int launch_slice(DeviceState* st, const uint32_t* d_in, uint32_t cols, uint32_t* d_out,
+int launch_slice(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols, uint32_t* d_out,
                  hipStream_t stream, bool dry);
 
 // Before tuning, squares of side >= 2^DIRECT_MIN_LOG run as their own fused

@@ -163,9 +163,10 @@ static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32
   const size_t lds_bytes = (size_t)PL::LDS_WORDS * sizeof(int32_t);
   const bool full = cols % PL::W == 0 && k == (uint32_t)PL::S;
   const bool row = layout == ZODA_LAYOUT_ROW;
-  // side 2048: the slice-pipelined kernel (32-column tiles) where it fits
-  if constexpr (PL::L == 11) {
-    if (slice_fits(k, cols, layout)) return launch_slice(st, d_in, cols, d_out, stream, dry);
+  // sides 2048 and 4096: the slice-pipelined kernel where it fits
+  if constexpr (PL::L == 11 || PL::L == 12) {
+    if (k == (uint32_t)PL::S && slice_fits(k, cols, layout))
+      return launch_slice(st, d_in, k, cols, d_out, stream, dry);
   }
   if constexpr (PL::PERSIST) {
     // Pipe addresses rows within a digit with 32-bit byte offsets; whole

@@ -1,6 +1,7 @@
-// Slice-pipelined encode of side 2048 on gfx950 (see slice_core.hpp for the
-// transform): one persistent 512-thread work-group per CU, 32-column tiles,
-// the middle of each tile's transform held in registers.
+// Slice-pipelined encode of sides 2048 and 4096 on gfx950 (see slice_core.hpp
+// for the transform and the two plans): one persistent 512-thread work-group
+// per CU, 32-column (side 2048) or 16-column (side 4096) tiles, the middle of
+// each tile's transform held in registers.
 //
 // Per CU the tiles run as a pipeline: eight steps in which the front of tile
 // i (P1, P2) and the back of tile i - 1 (P2', P1') share each step's LDS
@@ -11,7 +12,9 @@
 // P1 reads; P1' leaves its odd rows in a second staging buffer that the next
 // step stores.  A dword access per lane would take four times as many
 // vector-memory instructions, and the CU's address unit, busy with every
-// 128-byte row segment, then stalls the issuing waves.
+// row segment, then stalls the issuing waves.
+#include <type_traits>
+
 #include "common.hpp"
 #include "slice_core.hpp"
 
@@ -19,7 +22,6 @@ namespace zoda {
 
 namespace {
 
-using PL = slice::Plan2048;
 using slice::MulK;
 using slice::Tw2;
 
@@ -41,14 +43,15 @@ __device__ __forceinline__ void mem_prio(bool high) {
     __builtin_amdgcn_s_setprio(0);
 }
 
+template <class PL>
 struct Ctx {
-  slice::Tid t;
+  slice::Tid<PL> t;
   slice::Offs<PL> o;
   int wu;  // the wave index, uniform
   int32_t* lds;
-  const Tw2* t1w;
-  const double* t2;
-  const double* t3;
+  const typename PL::T1T* t1w;
+  const typename PL::TwT* t2;
+  const typename PL::TwT* t3;
   MulK k;
 };
 
@@ -59,11 +62,26 @@ struct Ctx {
 // branch and the compiler's counts of outstanding memory operations stay
 // exact -- a wave waits for the loads it needs, never for the stores issued
 // after them (loads and stores share one in-order counter on gfx9).
+//
+// A plan whose offsets pass 32 bits (WIDE) keeps tile pointers instead, and
+// each access builds its descriptor at its row (slice::TileRef).
+template <class PL>
+using BufT = std::conditional_t<PL::WIDE, slice::TileRef, GBuf>;
+
+template <class PL>
+__device__ __forceinline__ BufT<PL> tile_buf(const uint32_t* p, bool live) {
+  if constexpr (PL::WIDE)
+    return slice::TileRef{p, live};
+  else
+    return GBuf(p, live);
+}
+
+template <class PL>
 struct Bufs {
-  GBuf ibf, obf;  // front tile: input, output
-  GBuf ibn, obn;  // the next front tile
-  GBuf obb;       // back tile: output
-  GBuf obp;       // the tile of the odd rows left in the staging by the previous step
+  BufT<PL> ibf, obf;  // front tile: input, output
+  BufT<PL> ibn, obn;  // the next front tile
+  BufT<PL> obb;       // back tile: output
+  BufT<PL> obp;       // the tile of the odd rows left in the staging by the previous step
 };
 
 // Step SP (two phases):
@@ -75,8 +93,8 @@ struct Bufs {
 // instead of after it.  The two waves that share a SIMD (w and w + 4) take
 // opposite orders, so that while one of them waits for the address unit to
 // take its loads and stores, the other keeps the SIMD busy.
-template <unsigned EVM, int SP, bool MF>
-__device__ __forceinline__ void step(const Ctx& c, uint32_t c4, bool front, bool back, const Bufs& b,
+template <class PL, unsigned EVM, int SP, bool MF>
+__device__ __forceinline__ void step(const Ctx<PL>& c, uint32_t c4, bool front, bool back, const Bufs<PL>& b,
                                      U4 (&ld)[PL::QI], int32_t (&held)[PL::RB][PL::RC]) {
   // odd rows of step SP - 1 (of the previous iteration's last step for SP = 0)
   auto odd = [&] {
@@ -124,15 +142,16 @@ __device__ __forceinline__ void step(const Ctx& c, uint32_t c4, bool front, bool
     mem_prio(false);
   }
   lds_barrier();
-  if constexpr (SP + 1 < PL::STEPS) step<EVM, SP + 1, MF>(c, c4, front, back, b, ld, held);
+  if constexpr (SP + 1 < PL::STEPS) step<PL, EVM, SP + 1, MF>(c, c4, front, back, b, ld, held);
 }
 
 // The middle passes of the front tile; group KB copies step KB's even rows
 // (read again, 16 bytes a lane) when bit KB of EVM is set, so that the CU
 // keeps memory requests in flight through the passes that have no memory
 // work of their own.
-template <unsigned EVM, int KB>
-__device__ __forceinline__ void mid(const Ctx& c, uint32_t c4, const Bufs& b, int32_t (&held)[PL::RB][PL::RC]) {
+template <class PL, unsigned EVM, int KB>
+__device__ __forceinline__ void mid(const Ctx<PL>& c, uint32_t c4, const Bufs<PL>& b,
+                                    int32_t (&held)[PL::RB][PL::RC]) {
   U4 ev[PL::QI];
   constexpr bool COPY = (EVM >> KB) & 1u;
   if constexpr (COPY) slice::load_step<PL>(b.ibf, c.o, c.wu, KB, c4, ev);
@@ -140,9 +159,10 @@ __device__ __forceinline__ void mid(const Ctx& c, uint32_t c4, const Bufs& b, in
   if constexpr (COPY) {
     group_fence();
 #pragma unroll
-    for (int i = 0; i < PL::QI; i++) b.obf.store4(ev[i], c.o.ev, slice::Offs<PL>::s_out(KB, c.wu, i, c4));
+    for (int i = 0; i < PL::QI; i++)
+      slice::acc_store4(b.obf, ev[i], c.o.ev, slice::Offs<PL>::b_out(KB, c.wu), slice::Offs<PL>::o_out(KB, c.wu, i), c4);
   }
-  if constexpr (KB + 1 < PL::RB) mid<EVM, KB + 1>(c, c4, b, held);
+  if constexpr (KB + 1 < PL::RB) mid<PL, EVM, KB + 1>(c, c4, b, held);
 }
 
 // The waves of the upper half of the work-group run their steps memory first.
@@ -152,9 +172,9 @@ constexpr bool kSplitOrder = true;
 // prologue, the pipelined tiles and the epilogue.  Both orders are complete
 // functions, so no register state crosses the branch between them (at such a
 // join the two orders' register assignments would have to meet, and spill).
-template <unsigned EVM, bool MF>
+template <class PL, unsigned EVM, bool MF>
 __device__ __forceinline__ void run_tiles(const uint32_t* in, uint32_t* out, uint32_t cols,
-                                          uint32_t ntiles, const Ctx& c, int tid, int32_t* lds) {
+                                          uint32_t ntiles, const Ctx<PL>& c, int tid, int32_t* lds) {
   const uint32_t b = blockIdx.x, g = gridDim.x;
   const uint32_t mine = b < ntiles ? (ntiles - 1 - b) / g + 1 : 0;
   auto tile = [&](uint32_t i) { return xcd_tile(b + i * g, ntiles); };
@@ -169,7 +189,8 @@ __device__ __forceinline__ void run_tiles(const uint32_t* in, uint32_t* out, uin
   {
     const bool live = mine > 0;
     const uint32_t t0 = live ? tile(0) : 0u;
-    const GBuf ib(in + (size_t)t0 * PL::W, live), ob(out + (size_t)t0 * PL::W, live);
+    const BufT<PL> ib = tile_buf<PL>(in + (size_t)t0 * PL::W, live);
+    const BufT<PL> ob = tile_buf<PL>(out + (size_t)t0 * PL::W, live);
     slice::load_step<PL>(ib, c.o, c.wu, 0, cols * 4u, ld);
     slice::stage_step<PL, !(EVM & 1u)>(c.t, ob, c.o, 0, cols * 4u, ld, lds);
     slice::load_step<PL>(ib, c.o, c.wu, 1, cols * 4u, ld);
@@ -181,9 +202,9 @@ __device__ __forceinline__ void run_tiles(const uint32_t* in, uint32_t* out, uin
     const uint32_t tf = front ? tile(it) : 0u;
     const uint32_t tb = back ? tile(it - 1) : 0u;
     const uint32_t tn = next ? tile(it + 1) : 0u;
-    const Bufs bf{GBuf(in + (size_t)tf * PL::W, front), GBuf(out + (size_t)tf * PL::W, front),
-                  GBuf(in + (size_t)tn * PL::W, next),  GBuf(out + (size_t)tn * PL::W, next),
-                  GBuf(out + (size_t)tb * PL::W, back), GBuf(out + (size_t)tp * PL::W, prev)};
+    const Bufs<PL> bf{tile_buf<PL>(in + (size_t)tf * PL::W, front), tile_buf<PL>(out + (size_t)tf * PL::W, front),
+                      tile_buf<PL>(in + (size_t)tn * PL::W, next),  tile_buf<PL>(out + (size_t)tn * PL::W, next),
+                      tile_buf<PL>(out + (size_t)tb * PL::W, back), tile_buf<PL>(out + (size_t)tp * PL::W, prev)};
     uint32_t c4 = cols * 4u;
     asm volatile("" : "+s"(c4));
     // the thread's coordinates and addresses are rebuilt from an opaque copy of
@@ -191,74 +212,101 @@ __device__ __forceinline__ void run_tiles(const uint32_t* in, uint32_t* out, uin
     // free registers and spill
     int ti = tid;
     asm volatile("" : "+v"(ti));
-    const slice::Tid tt(ti);
-    const Ctx ci{tt, slice::Offs<PL>(tt, cols), c.wu, lds, c.t1w, c.t2, c.t3, c.k};
-    step<EVM, 0, MF>(ci, c4, front, back, bf, ld, held);
-    if (front) mid<EVM, 0>(ci, c4, bf, held);
+    const slice::Tid<PL> tt(ti);
+    const Ctx<PL> ci{tt, slice::Offs<PL>(tt, cols), c.wu, lds, c.t1w, c.t2, c.t3, c.k};
+    step<PL, EVM, 0, MF>(ci, c4, front, back, bf, ld, held);
+    if (front) mid<PL, EVM, 0>(ci, c4, bf, held);
     tp = tb;
   }
   // the odd rows of the last tile's last step
-  const GBuf ob(out + (size_t)tp * PL::W, mine > 0);
+  const BufT<PL> ob = tile_buf<PL>(out + (size_t)tp * PL::W, mine > 0);
   slice::store_odd<PL>(c.t, ob, c.o, PL::STEPS - 1, cols * 4u, lds);
 }
 
-template <unsigned EVM>
+template <class PL, unsigned EVM>
 __global__ __launch_bounds__(PL::NT) void encode_slice(const uint32_t* __restrict__ in,
                                                        uint32_t* __restrict__ out, uint32_t cols,
-                                                       uint32_t ntiles, const Tw2* __restrict__ t1,
-                                                       const double* __restrict__ t2g,
-                                                       const double* __restrict__ t3g) {
+                                                       uint32_t ntiles,
+                                                       const typename PL::T1T* __restrict__ t1,
+                                                       const typename PL::TwT* __restrict__ t2g,
+                                                       const typename PL::TwT* __restrict__ t3g) {
+  using TwT = typename PL::TwT;
   extern __shared__ int32_t lds[];
   const int tid = (int)threadIdx.x;
-  double* t2 = reinterpret_cast<double*>(lds + 4 * PL::XWORDS);
-  double* t3 = t2 + PL::T2_WORDS;
+  TwT* t2 = reinterpret_cast<TwT*>(lds + 4 * PL::XWORDS);
+  TwT* t3 = t2 + PL::T2_WORDS;
   for (int i = tid; i < PL::T2_WORDS; i += PL::NT) t2[i] = t2g[i];
   for (int i = tid; i < PL::T3_WORDS; i += PL::NT) t3[i] = t3g[i];
-  const slice::Tid t(tid);
+  const slice::Tid<PL> t(tid);
   const int wu = __builtin_amdgcn_readfirstlane(t.w);
-  const Ctx c{t, slice::Offs<PL>(t, cols), wu, lds, t1 + wu * PL::RA, t2, t3, slice::make_mulk()};
+  constexpr int T1_PER_WAVE = PL::T1_PAIRS / PL::RB;  // the P1 twiddles of nB = wave
+  const Ctx<PL> c{t, slice::Offs<PL>(t, cols), wu, lds, t1 + wu * T1_PER_WAVE, t2, t3, slice::make_mulk()};
   if (kSplitOrder && wu >= PL::NT / 128)
-    run_tiles<EVM, true>(in, out, cols, ntiles, c, tid, lds);
+    run_tiles<PL, EVM, true>(in, out, cols, ntiles, c, tid, lds);
   else
-    run_tiles<EVM, false>(in, out, cols, ntiles, c, tid, lds);
+    run_tiles<PL, EVM, false>(in, out, cols, ntiles, c, tid, lds);
+}
+
+// Largest per-lane byte offset of a plan's accesses (the odd output row of the
+// last lane row, its last column quad) below 2^32; a narrow plan's uniform
+// offsets too.
+template <class PL>
+bool slice_plan_fits(uint32_t cols) {
+  if (cols == 0 || cols % PL::W != 0) return false;
+  int max_lane_row = 0;
+  for (int g = 0; g < 64; g++) max_lane_row = std::max(max_lane_row, PL::lane_row(g & 15));
+  // a step's accesses share a base row; the offsets of its last access
+  const uint32_t last_off = slice::Offs<PL>::o_out(0, 0, PL::QI - 1);
+  const uint64_t lane = ((uint64_t)(last_off + 2 * max_lane_row + 1) * cols + PL::W) * 4;
+  if (PL::WIDE) return lane < (1ull << 32);
+  return ((uint64_t)(2 * PL::S - 1) * cols + PL::W) * 4 < (1ull << 32);
 }
 
 }  // namespace
 
 bool slice_fits(uint32_t k, uint32_t cols, int layout) {
-  return k == (uint32_t)PL::S && layout == ZODA_LAYOUT_ROW && cols > 0 && cols % PL::W == 0 &&
-         ((uint64_t)(2 * PL::S - 1) * cols + PL::W) * 4 < (1ull << 32);
+  if (layout != ZODA_LAYOUT_ROW) return false;
+  if (k == (uint32_t)slice::Plan2048::S) return slice_plan_fits<slice::Plan2048>(cols);
+  if (k == (uint32_t)slice::Plan4096::S) return slice_plan_fits<slice::Plan4096>(cols);
+  return false;
 }
 
-static int get_slice_tables(DeviceState* st, const Tw2** t1, const double** t2, const double** t3) {
+namespace {
+
+template <class PL>
+int get_slice_tables(DeviceState* st, int slot, const typename PL::T1T** t1, const typename PL::TwT** t2,
+                     const typename PL::TwT** t3) {
+  using T1T = typename PL::T1T;
+  using TwT = typename PL::TwT;
   std::lock_guard<std::recursive_mutex> lock(st->mu);
-  constexpr size_t bytes = PL::T1_PAIRS * sizeof(Tw2) + (PL::T2_WORDS + PL::T3_WORDS) * sizeof(double);
-  if (!st->slice_tables) {
+  constexpr size_t bytes = PL::T1_PAIRS * sizeof(T1T) + (PL::T2_WORDS + PL::T3_WORDS) * sizeof(TwT);
+  if (!st->slice_tables[slot]) {
     std::vector<char> host(bytes);
-    Tw2* h1 = reinterpret_cast<Tw2*>(host.data());
-    double* h2 = reinterpret_cast<double*>(h1 + PL::T1_PAIRS);
+    T1T* h1 = reinterpret_cast<T1T*>(host.data());
+    TwT* h2 = reinterpret_cast<TwT*>(h1 + PL::T1_PAIRS);
     slice::build_slice_tables<PL>(h1, h2, h2 + PL::T2_WORDS);
     void* d = nullptr;
-    ZODA_HIP(hipMalloc(&d, bytes));
+    if (const hipError_t ea = hipMalloc(&d, bytes)) return alloc_fail(ea, "slice tables");
     hipError_t e = hipMemcpy(d, host.data(), bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       (void)hipFree(d);
       return fail((int)e, "copying slice tables failed: %s", hipGetErrorString(e));
     }
-    st->slice_tables = d;
+    st->slice_tables[slot] = d;
   }
-  *t1 = static_cast<const Tw2*>(st->slice_tables);
-  *t2 = reinterpret_cast<const double*>(*t1 + PL::T1_PAIRS);
+  *t1 = static_cast<const T1T*>(st->slice_tables[slot]);
+  *t2 = reinterpret_cast<const TwT*>(*t1 + PL::T1_PAIRS);
   *t3 = *t2 + PL::T2_WORDS;
   return 0;
 }
 
-int launch_slice(DeviceState* st, const uint32_t* d_in, uint32_t cols, uint32_t* d_out,
-                 hipStream_t stream, bool dry) {
-  const Tw2* t1 = nullptr;
-  const double *t2 = nullptr, *t3 = nullptr;
-  ZODA_TRY(get_slice_tables(st, &t1, &t2, &t3));
-  const void* fn = (const void*)encode_slice<PL::EVEN_MID_MASK>;
+template <class PL>
+int launch_slice_plan(DeviceState* st, int slot, const uint32_t* d_in, uint32_t cols, uint32_t* d_out,
+                      hipStream_t stream, bool dry) {
+  const typename PL::T1T* t1 = nullptr;
+  const typename PL::TwT *t2 = nullptr, *t3 = nullptr;
+  ZODA_TRY(get_slice_tables<PL>(st, slot, &t1, &t2, &t3));
+  const void* fn = (const void*)encode_slice<PL, PL::EVEN_MID_MASK>;
   uint32_t grid = 0;
   ZODA_TRY(kernel_setup(st, fn, PL::NT, PL::LDS_BYTES, true, &grid));
   if (dry) return 0;
@@ -268,6 +316,17 @@ int launch_slice(DeviceState* st, const uint32_t* d_in, uint32_t cols, uint32_t*
                   (void*)&t1,   (void*)&t2,    (void*)&t3};
   ZODA_HIP(hipLaunchKernel(fn, dim3(grid), dim3(PL::NT), args, PL::LDS_BYTES, stream));
   return 0;
+}
+
+}  // namespace
+
+int launch_slice(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols, uint32_t* d_out,
+                 hipStream_t stream, bool dry) {
+  if (k == (uint32_t)slice::Plan2048::S)
+    return launch_slice_plan<slice::Plan2048>(st, 0, d_in, cols, d_out, stream, dry);
+  if (k == (uint32_t)slice::Plan4096::S)
+    return launch_slice_plan<slice::Plan4096>(st, 1, d_in, cols, d_out, stream, dry);
+  return fail(ZODA_EINVAL, "no slice plan for side %u", k);
 }
 
 }  // namespace zoda

@@ -207,6 +207,35 @@ ZHD void dft(double (&x)[1 << R], const MulK& k, const Hook& hook = Hook()) {
   for (int i = 0; i < N; i++) x[i] = z[i];
 }
 
+// The first radix-2 stage's twiddles of a DFT of 2^R values split between two
+// threads: x_n (n < 2^(R-1)) times w_(2^R)^(-+n) where hd = 1 (odd outputs),
+// times 1 where hd = 0; the per-lane constant is 1 + hd (w - 1), exact, and
+// its quotient by p one multiply.  In batches of four, as in dft().
+template <int R, bool INV>
+ZHD void split_input_twist(double (&x)[1 << (R - 1)], double hd, const MulK& k) {
+  constexpr int H = 1 << (R - 1);
+  constexpr int D = INV ? 1 : 0;
+#pragma unroll
+  for (int n0 = 1; n0 < H; n0 += 4) {
+    double w[4], wq[4], y[4], c[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (n0 + q < H) {
+        w[q] = fmad(hd, kRoots.w[D][(n0 + q) << (7 - R)] - 1.0, 1.0);
+        wq[q] = w[q] * PINV;
+      }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (n0 + q < H) y[q] = fmad(x[n0 + q], wq[q], k.m);
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (n0 + q < H) c[q] = fmad(y[q], k.kd, k.kr);
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (n0 + q < H) x[n0 + q] = fmad(x[n0 + q], w[q], -c[q]) - y[q];
+  }
+}
+
 // n values times n twiddles, as int32 (mul_mem), in batches of four.
 template <int NV, class TW>
 ZHD void mul_mem_batch(const double (&x)[NV], const TW& tw, int32_t (&r)[NV], const MulK& k) {
@@ -235,6 +264,10 @@ ZHD void mul_mem_batch(const double (&x)[NV], const TW& tw, int32_t (&r)[NV], co
 
 struct Plan2048 {
   static constexpr int LRA = 4, LRB = 3, LRC = 4;
+  static constexpr bool SPLIT = false;  // P1 / P1' DFTs whole in one thread
+  static constexpr bool WIDE = false;   // every output offset within 32 bits of the tile base
+  using TwT = double;                   // P2 / P2' twiddles in LDS
+  using T1T = Tw2;                      // P1 twiddles, global memory
   static constexpr int RA = 1 << LRA, RB = 1 << LRB, RC = 1 << LRC;
   static constexpr int L = LRA + LRB + LRC, S = 1 << L;
   static constexpr int W = 32, NT = 512, WAVES = NT / 64;
@@ -269,155 +302,295 @@ struct Plan2048 {
   static ZHD int xi(int s, int d, int ka, int col) { return ((s * RB + d) * RA + ka) * W + col; }
   // staging index (IN / ODD) of (slice s, digit a in {nA, j3}, digit b in {nB, j2}, column)
   static ZHD int si(int s, int a, int b, int col) { return ((s * RA + a) * RB + b) * W + col; }
+  // Threads: wave w; task (P1, P1') slice s = half-wave, column = lane mod 32,
+  // nB (resp. j2) = w; owner (P2, middle, P2') the same column and kA = 2 w + s;
+  // 16-byte access lane: column quad q, row g.
+  static ZHD void decode(int tid, int& w, int& s, int& hh, int& col, int& ocol, int& oka, int& q, int& g) {
+    w = tid >> 6;
+    s = (tid >> 5) & 1;
+    hh = 0;
+    col = tid & 31;
+    ocol = col;
+    oka = 2 * w + s;
+    q = tid & 7;
+    g = (tid >> 3) & 7;
+  }
+  // Access i of wave w in step sp covers input rows (2 sp + s) + RC g + RC RB a
+  // (slice s = w / 4, a = (4 w + i) mod 16, nB = g), four columns 4q..4q+3.
+  static ZHD int slice_of(int w) { return w / (WAVES / SPS); }
+  static ZHD int a_of(int w, int i) { return (QI * w + i) % RA; }
+  static ZHD int lane_row(int g) { return RC * g; }
+  static ZHD int acc_row(int sp, int w, int i) { return SPS * sp + slice_of(w) + RC * RB * a_of(w, i); }
+  static ZHD int acc_lds(int w, int i, int g, int q) { return si(slice_of(w), a_of(w, i), g, 4 * q); }
+};
+
+// Side 4096: 32 x 8 x 16 digits, 16-column tiles (64-byte row segments, as
+// wide as a tile can be with 128 held values per thread).  The radix-32 P1 and
+// P1' DFTs are split between two threads: a first radix-2 stage on all 32
+// inputs, then one radix-16 DFT per thread for the even (hh = 0) or odd
+// (hh = 1) outputs.  Outputs pass 32-bit offsets (16 GiB), so every access
+// addresses from a per-access base.
+struct Plan4096 {
+  static constexpr int LRA = 5, LRB = 3, LRC = 4;
+  static constexpr bool SPLIT = true;
+  static constexpr bool WIDE = true;
+  using TwT = uint32_t;  // u32 in LDS: as doubles the two tables would not fit
+  struct T1T;            // defined below
+  static constexpr int RA = 1 << LRA, RB = 1 << LRB, RC = 1 << LRC;
+  static constexpr int L = LRA + LRB + LRC, S = 1 << L;
+  static constexpr int W = 16, NT = 512, WAVES = NT / 64;
+  static constexpr int SPS = 2;
+  static constexpr int STEPS = RC / SPS;
+  static_assert(RB == WAVES, "P1/P1': the wave index is nB (resp. j2)");
+  static_assert(W * SPS * 2 == 64, "a wave holds column x slice x half tasks");
+  static_assert(W * RA == NT, "owners (column, kA)");
+  static constexpr int QI = 4;
+  static_assert(QI * 16 * WAVES == SPS * RA * RB, "one 16-byte access per lane covers a step");
+  static constexpr unsigned EVEN_MID_MASK = 0x55u;
+  static constexpr ZHD bool even_in_mid(int s) { return (EVEN_MID_MASK >> s) & 1u; }
+  static_assert(EVEN_MID_MASK == 0 || RB == STEPS, "one step of even rows per middle group");
+  static constexpr int XWORDS = SPS * RB * RA * W;
+  static constexpr int T1_PAIRS = RB * RA / 2;      // [nB][m] (base, diff) pairs, global memory
+  static constexpr int T2_WORDS = RC * RA * RB;     // [c][kA][kB] u32 in LDS
+  static constexpr int T3_WORDS = RC * RA * RB;     // [j1][kA][j2] u32 in LDS
+  static constexpr int A_OFF = 0, B_OFF = XWORDS, IN_OFF = 2 * XWORDS, ODD_OFF = 3 * XWORDS;
+  static constexpr int LDS_BYTES = 4 * XWORDS * 4 + (T2_WORDS + T3_WORDS) * 4;
+  static_assert(LDS_BYTES <= 160 * 1024, "LDS of one CU");
+  static ZHD int xi(int s, int d, int ka, int col) { return ((s * RB + d) * RA + ka) * W + col; }
+  static ZHD int si(int s, int a, int b, int col) { return ((s * RA + a) * RB + b) * W + col; }
+  // Threads: wave w; task slice s, half hh, column (lane bits 0-3, 4, 5),
+  // nB (resp. j2) = w; owner column = lane mod 16, kA = 4 w + lane / 16;
+  // 16-byte access lane: column quad q (bits 0-1), row g (bits 2-5).
+  static ZHD void decode(int tid, int& w, int& s, int& hh, int& col, int& ocol, int& oka, int& q, int& g) {
+    w = tid >> 6;
+    s = (tid >> 4) & 1;
+    hh = (tid >> 5) & 1;
+    col = tid & 15;
+    ocol = col;
+    oka = 4 * w + ((tid >> 4) & 3);
+    q = tid & 3;
+    g = (tid >> 2) & 15;
+  }
+  // Access i of wave w in step sp covers input rows (2 sp + s) + RC nB +
+  // RC RB nA with slice s = w / 4, nA = 2 a + g / 8 (a = (4 w + i) mod 16),
+  // nB = g mod 8, four columns 4q..4q+3.
+  static ZHD int slice_of(int w) { return w / (WAVES / SPS); }
+  static ZHD int a_of(int w, int i) { return (QI * w + i) % (RA / 2); }
+  static ZHD int lane_row(int g) { return RC * (g & 7) + RC * RB * (g >> 3); }
+  static ZHD int acc_row(int sp, int w, int i) { return SPS * sp + slice_of(w) + 2 * RC * RB * a_of(w, i); }
+  static ZHD int acc_lds(int w, int i, int g, int q) {
+    return si(slice_of(w), 2 * a_of(w, i) + (g >> 3), g & 7, 4 * q);
+  }
+};
+
+// P1 twiddles of a split plan for one wave (nB): output m of half hh is kA =
+// 2m + hh, twiddle w_(RA RB)^-(nB kA) = base + hh * diff (exact integers).
+struct alignas(16) Plan4096::T1T {
+  double base;
+  double diff;
 };
 
 // Host-built twiddle tables.
-//   t1[nB][kA]     = w_(RA RB)^-(nB kA)                  (w, w/p) pairs
+//   t1[nB][kA]     = w_(RA RB)^-(nB kA)                  (w, w/p) pairs; for a
+//                    split plan [nB][m] = (T(2m), T(2m+1) - T(2m))
 //   t2[c][kA][kB]  = s^-1 w_2s^((1 - 2c)(kA + RA kB))    w only
 //   t3[j1][kA][j2] = w_s^(kA (j1 + RC j2))               w only
 template <class PL>
-inline void build_slice_tables(Tw2* t1, double* t2, double* t3) {
+inline void build_slice_tables(typename PL::T1T* t1, typename PL::TwT* t2, typename PL::TwT* t3) {
   const uint32_t ws = root_of_unity(PL::L), w2s = root_of_unity(PL::L + 1);
   const uint32_t wm = pow_mod_host(ws, PL::RC);  // w_(RA RB)
   const uint32_t wmi = inv_mod_host(wm);
   const uint32_t sinv = inv_mod_host((uint32_t)PL::S);
-  for (int nb = 0; nb < PL::RB; nb++)
-    for (int ka = 0; ka < PL::RA; ka++) {
-      const uint32_t v = pow_mod_host(wmi, (uint64_t)nb * ka);
-      t1[nb * PL::RA + ka] = Tw2{(double)v, (double)v / PD};
+  auto t1v = [&](int nb, int ka) { return pow_mod_host(wmi, (uint64_t)nb * ka); };
+  for (int nb = 0; nb < PL::RB; nb++) {
+    if constexpr (PL::SPLIT) {
+      for (int m = 0; m < PL::RA / 2; m++) {
+        const double a = (double)t1v(nb, 2 * m), b = (double)t1v(nb, 2 * m + 1);
+        t1[nb * (PL::RA / 2) + m] = typename PL::T1T{a, b - a};
+      }
+    } else {
+      for (int ka = 0; ka < PL::RA; ka++) {
+        const uint32_t v = t1v(nb, ka);
+        t1[nb * PL::RA + ka] = Tw2{(double)v, (double)v / PD};
+      }
     }
+  }
   const int64_t ord2s = 2ll * PL::S;
   for (int c = 0; c < PL::RC; c++)
     for (int ka = 0; ka < PL::RA; ka++)
       for (int kb = 0; kb < PL::RB; kb++) {
         const int64_t e = (int64_t)(1 - 2 * c) * (ka + PL::RA * kb);
         const uint64_t em = (uint64_t)(((e % ord2s) + ord2s) % ord2s);
-        t2[(c * PL::RA + ka) * PL::RB + kb] = (double)mul_mod_host(sinv, pow_mod_host(w2s, em));
+        t2[(c * PL::RA + ka) * PL::RB + kb] = (typename PL::TwT)mul_mod_host(sinv, pow_mod_host(w2s, em));
       }
   for (int j1 = 0; j1 < PL::RC; j1++)
     for (int ka = 0; ka < PL::RA; ka++)
       for (int j2 = 0; j2 < PL::RB; j2++)
         t3[(j1 * PL::RA + ka) * PL::RB + j2] =
-            (double)pow_mod_host(ws, (uint64_t)ka * (j1 + PL::RC * j2) % PL::S);
+            (typename PL::TwT)pow_mod_host(ws, (uint64_t)ka * (j1 + PL::RC * j2) % PL::S);
 }
 
 // A twiddle w read as a double with its quotient w/p formed by one multiply.
+template <class T = double>
 struct TwW {
-  const double* p;
+  const T* p;
   ZHD Tw2 operator()(int i) const {
-    const double w = p[i];
+    const double w = (double)p[i];
     return Tw2{w, w * PINV};
   }
 };
 
-// Thread coordinates: wave w, half h, column col (owner kA = 2 w + h), and
-// for the 16-byte accesses column quad q and row g.
+// Thread coordinates (the plan's decode): wave w; as a P1 / P1' task slice
+// s, DFT half hh and column col; as an owner column ocol and kA oka; for the
+// 16-byte accesses column quad q and row g.
+template <class PL>
 struct Tid {
-  int w, h, col, q, g;
-  ZHD explicit Tid(int tid)
-      : w(tid >> 6), h((tid >> 5) & 1), col(tid & 31), q(tid & 7), g((tid >> 3) & 7) {}
-  ZHD int ka() const { return 2 * w + h; }
+  int w, s, hh, col, ocol, oka, q, g;
+  ZHD explicit Tid(int tid) { PL::decode(tid, w, s, hh, col, ocol, oka, q, g); }
 };
 
-// Byte offsets of the 16-byte accesses (raw-buffer form: a per-thread
-// voffset plus a uniform soffset, from the tile's first column).  Access i of
-// step sp reads input row 2 sp + s + RC g + RC RB a (slice s = w / 4, a =
-// (4 w + i) mod 16, nB = g), four columns 4q..4q+3; the even output row is
-// twice that; the odd row for P1' value (j1 = 2 sp + s, j2 = g, j3 = a) is
-// 2 (2 sp + s + RC g + RC RB a) + 1.
+// A tile's columns in a row-major matrix, for plans whose offsets pass 32
+// bits: each access builds its own buffer descriptor at its row.
+struct TileRef {
+  const uint32_t* p;
+  bool live;
+};
+
+// 16-byte accesses at a per-lane byte offset voff plus the uniform rows
+// base + off: a 32-bit soffset of a tile's buffer, or, for a TileRef, a
+// descriptor at row `base` (the same for every access of a step, so one per
+// step) and a 32-bit soffset of `off` rows.
+ZHD U4 acc_load4(const GBuf& b, uint32_t voff, uint32_t base, uint32_t off, uint32_t c4) {
+  return b.load4(voff, (base + off) * c4);
+}
+ZHD void acc_store4(const GBuf& b, const U4& v, uint32_t voff, uint32_t base, uint32_t off, uint32_t c4) {
+  b.store4(v, voff, (base + off) * c4);
+}
+ZHD U4 acc_load4(const TileRef& b, uint32_t voff, uint32_t base, uint32_t off, uint32_t c4) {
+  return GBuf(b.p + (size_t)base * (c4 / 4u), b.live).load4(voff, off * c4);
+}
+ZHD void acc_store4(const TileRef& b, const U4& v, uint32_t voff, uint32_t base, uint32_t off, uint32_t c4) {
+  GBuf(const_cast<uint32_t*>(b.p) + (size_t)base * (c4 / 4u), b.live).store4(v, voff, off * c4);
+}
+
+// Byte offsets of the 16-byte accesses: a per-thread voffset from the tile's
+// first column plus the uniform row of access i of step sp (PL::acc_row):
+// input row acc_row + lane_row, even output row twice that, odd output row
+// twice that plus one (the odd row of P1' value j1 + RC j2 + RC RB j3 has the
+// same digits as input row c + RC nB + RC RB nA).
 template <class PL>
 struct Offs {
   uint32_t in, ev, od;
-  ZHD Offs(const Tid& t, uint32_t cols) {
-    const uint32_t r = (uint32_t)(PL::RC * t.g);
+  ZHD Offs(const Tid<PL>& t, uint32_t cols) {
+    const uint32_t r = (uint32_t)PL::lane_row(t.g);
     in = (r * cols + 4u * (uint32_t)t.q) * 4u;
     ev = (2 * r * cols + 4u * (uint32_t)t.q) * 4u;
     od = ((2 * r + 1) * cols + 4u * (uint32_t)t.q) * 4u;
   }
-  static ZHD int slice_of(int w) { return w / (PL::WAVES / PL::SPS); }
-  static ZHD int a_of(int w, int i) { return (PL::QI * w + i) % PL::RA; }
-  // c4 = cols * 4, the byte stride of a row (the kernel hides its value per
-  // tile, so these uniform offsets are computed where they are used instead
-  // of being hoisted out of the tile loop into scalar registers that spill)
-  static ZHD uint32_t s_in(int sp, int w, int i, uint32_t c4) {
-    return (uint32_t)(PL::SPS * sp + slice_of(w) + PL::RC * PL::RB * a_of(w, i)) * c4;
-  }
-  static ZHD uint32_t s_out(int sp, int w, int i, uint32_t c4) { return 2u * s_in(sp, w, i, c4); }
+  // uniform rows of access i (c4 = cols * 4, the byte stride of a row, is
+  // applied where the offsets are used: the kernel hides its value per tile,
+  // so they are not hoisted out of the tile loop into scalar registers that
+  // spill)
+  static ZHD uint32_t r_in(int sp, int w, int i) { return (uint32_t)PL::acc_row(sp, w, i); }
+  static ZHD uint32_t r_out(int sp, int w, int i) { return 2u * r_in(sp, w, i); }
+  // the same split into the step's first access and the offset of access i
+  static ZHD uint32_t b_in(int sp, int w) { return r_in(sp, w, 0); }
+  static ZHD uint32_t o_in(int sp, int w, int i) { return r_in(sp, w, i) - r_in(sp, w, 0); }
+  static ZHD uint32_t b_out(int sp, int w) { return 2u * b_in(sp, w); }
+  static ZHD uint32_t o_out(int sp, int w, int i) { return 2u * o_in(sp, w, i); }
   // staging index (IN / ODD) of access i's four columns
-  static ZHD int lds(const Tid& t, int i) { return PL::si(slice_of(t.w), a_of(t.w, i), t.g, 4 * t.q); }
+  static ZHD int lds(const Tid<PL>& t, int i) { return PL::acc_lds(t.w, i, t.g, t.q); }
 };
 
-// The 16-byte loads of one step's input rows.
-template <class PL>
-ZHD void load_step(const GBuf& ib, const Offs<PL>& o, int w, int sp, uint32_t c4, U4 (&r)[PL::QI]) {
+// The 16-byte loads of one step's input rows (B: GBuf or TileRef).
+template <class PL, class B>
+ZHD void load_step(const B& ib, const Offs<PL>& o, int w, int sp, uint32_t c4, U4 (&r)[PL::QI]) {
 #pragma unroll
-  for (int i = 0; i < PL::QI; i++) r[i] = ib.load4(o.in, Offs<PL>::s_in(sp, w, i, c4));
+  for (int i = 0; i < PL::QI; i++) r[i] = acc_load4(ib, o.in, Offs<PL>::b_in(sp, w), Offs<PL>::o_in(sp, w, i), c4);
 }
 
 // A step's loaded rows into the IN staging and, with EVEN, to the even
 // output rows.
-template <class PL, bool EVEN = true>
-ZHD void stage_step(const Tid& t, const GBuf& ob, const Offs<PL>& o, int sp, uint32_t c4,
+template <class PL, bool EVEN = true, class B = GBuf>
+ZHD void stage_step(const Tid<PL>& t, const B& ob, const Offs<PL>& o, int sp, uint32_t c4,
                     const U4 (&r)[PL::QI], int32_t* __restrict__ lds) {
 #pragma unroll
   for (int i = 0; i < PL::QI; i++) {
-    if constexpr (EVEN) ob.store4(r[i], o.ev, Offs<PL>::s_out(sp, t.w, i, c4));
+    if constexpr (EVEN) acc_store4(ob, r[i], o.ev, Offs<PL>::b_out(sp, t.w), Offs<PL>::o_out(sp, t.w, i), c4);
     *reinterpret_cast<U4*>(lds + PL::IN_OFF + Offs<PL>::lds(t, i)) = r[i];
   }
 }
 
-// The even output rows of step sp: the input rows read again (16-byte
-// accesses) and written back out.
-template <class PL>
-ZHD void copy_even(const GBuf& ib, const GBuf& ob, const Offs<PL>& o, int w, int sp, uint32_t c4) {
-  U4 r[PL::QI];
+// The even output rows of access i of step sp (copied from registers).
+template <class PL, class B>
+ZHD void store_even(const B& ob, const Offs<PL>& o, int w, int sp, uint32_t c4, const U4 (&r)[PL::QI]) {
 #pragma unroll
-  for (int i = 0; i < PL::QI; i++) r[i] = ib.load4(o.in, Offs<PL>::s_in(sp, w, i, c4));
-#pragma unroll
-  for (int i = 0; i < PL::QI; i++) ob.store4(r[i], o.ev, Offs<PL>::s_out(sp, w, i, c4));
+  for (int i = 0; i < PL::QI; i++) acc_store4(ob, r[i], o.ev, Offs<PL>::b_out(sp, w), Offs<PL>::o_out(sp, w, i), c4);
 }
 
 // The odd rows P1' left in the ODD staging, as 16-byte stores.
-template <class PL>
-ZHD void store_odd(const Tid& t, const GBuf& ob, const Offs<PL>& o, int sp, uint32_t c4,
+template <class PL, class B>
+ZHD void store_odd(const Tid<PL>& t, const B& ob, const Offs<PL>& o, int sp, uint32_t c4,
                    const int32_t* __restrict__ lds) {
   U4 v[PL::QI];
 #pragma unroll
   for (int i = 0; i < PL::QI; i++) v[i] = *reinterpret_cast<const U4*>(lds + PL::ODD_OFF + Offs<PL>::lds(t, i));
 #pragma unroll
-  for (int i = 0; i < PL::QI; i++) ob.store4(v[i], o.od, Offs<PL>::s_out(sp, t.w, i, c4));
+  for (int i = 0; i < PL::QI; i++) acc_store4(ob, v[i], o.od, Offs<PL>::b_out(sp, t.w), Offs<PL>::o_out(sp, t.w, i), c4);
 }
 
-// P1: INTT over nA of the thread's (nB = w, c = 2 sp + h) values (IN) -> A.
+// P1: INTT over nA of the thread's (nB = w, c = 2 sp + s) values (IN) -> A.
 template <class PL>
-ZHD void p1(const Tid& t, int32_t* __restrict__ lds, const Tw2* __restrict__ t1w, const MulK& k) {
+ZHD void p1(const Tid<PL>& t, int32_t* __restrict__ lds, const typename PL::T1T* __restrict__ t1w,
+            const MulK& k) {
   group_fence();
   const uint32_t* in = reinterpret_cast<const uint32_t*>(lds + PL::IN_OFF);
-  double x[PL::RA];
+  if constexpr (!PL::SPLIT) {
+    double x[PL::RA];
 #pragma unroll
-  for (int a = 0; a < PL::RA; a++) x[a] = (double)in[PL::si(t.h, a, t.w, t.col)];
-  dft<PL::LRA, true>(x, k);
-  int32_t r[PL::RA];
-  mul_mem_batch(x, [&](int i) -> const Tw2& { return t1w[i]; }, r, k);
+    for (int a = 0; a < PL::RA; a++) x[a] = (double)in[PL::si(t.s, a, t.w, t.col)];
+    dft<PL::LRA, true>(x, k);
+    int32_t r[PL::RA];
+    mul_mem_batch(x, [&](int i) -> const Tw2& { return t1w[i]; }, r, k);
 #pragma unroll
-  for (int ka = 0; ka < PL::RA; ka++) lds[PL::A_OFF + PL::xi(t.h, t.w, ka, t.col)] = r[ka];
+    for (int ka = 0; ka < PL::RA; ka++) lds[PL::A_OFF + PL::xi(t.s, t.w, ka, t.col)] = r[ka];
+  } else {
+    // Split radix-2^LRA DFT: kA = 2m + hh.  First stage u_n = x_n + x_(n+H),
+    // v_n = (x_n - x_(n+H)) w_RA^-n (hh selects u or v per lane), then a
+    // radix-H DFT; the output twiddle w_(RA RB)^-(nB kA) per lane.
+    constexpr int H = PL::RA / 2;
+    const double hd = (double)t.hh, sgn = 1.0 - 2.0 * hd;
+    double x[H];
+#pragma unroll
+    for (int n = 0; n < H; n++)
+      x[n] = fmad((double)in[PL::si(t.s, n + H, t.w, t.col)], sgn, (double)in[PL::si(t.s, n, t.w, t.col)]);
+    split_input_twist<PL::LRA, true>(x, hd, k);
+    dft<PL::LRA - 1, true>(x, k);
+    int32_t r[H];
+    mul_mem_batch(x, [&](int m) -> Tw2 {
+      const double w = fmad(hd, t1w[m].diff, t1w[m].base);
+      return Tw2{w, w * PINV};
+    }, r, k);
+#pragma unroll
+    for (int m = 0; m < H; m++) lds[PL::A_OFF + PL::xi(t.s, t.w, 2 * m + t.hh, t.col)] = r[m];
+  }
 }
 
 // P2: INTT over nB for the owner's kA and the two slices c = 2 SP + s.
 template <class PL, int SP>
-ZHD void p2(const Tid& t, const int32_t* __restrict__ lds, const double* __restrict__ t2,
+ZHD void p2(const Tid<PL>& t, const int32_t* __restrict__ lds, const typename PL::TwT* __restrict__ t2,
             int32_t (&held)[PL::RB][PL::RC], const MulK& k) {
-  const int ka = t.ka();
+  const int ka = t.oka;
 #pragma unroll
   for (int s = 0; s < PL::SPS; s++) {
     group_fence();
     constexpr int c0 = PL::SPS * SP;
     double x[PL::RB];
 #pragma unroll
-    for (int nb = 0; nb < PL::RB; nb++) x[nb] = (double)lds[PL::A_OFF + PL::xi(s, nb, ka, t.col)];
+    for (int nb = 0; nb < PL::RB; nb++) x[nb] = (double)lds[PL::A_OFF + PL::xi(s, nb, ka, t.ocol)];
     dft<PL::LRB, true>(x, k);
     int32_t r[PL::RB];
-    mul_mem_batch(x, TwW{t2 + ((c0 + s) * PL::RA + ka) * PL::RB}, r, k);
+    mul_mem_batch(x, TwW<typename PL::TwT>{t2 + ((c0 + s) * PL::RA + ka) * PL::RB}, r, k);
 #pragma unroll
     for (int kb = 0; kb < PL::RB; kb++) held[kb][c0 + s] = r[kb];
   }
@@ -464,9 +637,9 @@ ZHD void mid_group(int32_t (&held)[PL::RB][PL::RC], const MulK& k) {
 
 // P2': NTT over kB for the owner's kA and the two slices j1 = 2 SP + s -> B.
 template <class PL, int SP>
-ZHD void p2b(const Tid& t, const int32_t (&held)[PL::RB][PL::RC], int32_t* __restrict__ lds,
-             const double* __restrict__ t3, const MulK& k) {
-  const int ka = t.ka();
+ZHD void p2b(const Tid<PL>& t, const int32_t (&held)[PL::RB][PL::RC], int32_t* __restrict__ lds,
+             const typename PL::TwT* __restrict__ t3, const MulK& k) {
+  const int ka = t.oka;
 #pragma unroll
   for (int s = 0; s < PL::SPS; s++) {
     group_fence();
@@ -476,24 +649,39 @@ ZHD void p2b(const Tid& t, const int32_t (&held)[PL::RB][PL::RC], int32_t* __res
     for (int kb = 0; kb < PL::RB; kb++) x[kb] = (double)held[kb][j0 + s];
     dft<PL::LRB, false>(x, k);
     int32_t r[PL::RB];
-    mul_mem_batch(x, TwW{t3 + ((j0 + s) * PL::RA + ka) * PL::RB}, r, k);
+    mul_mem_batch(x, TwW<typename PL::TwT>{t3 + ((j0 + s) * PL::RA + ka) * PL::RB}, r, k);
 #pragma unroll
-    for (int j2 = 0; j2 < PL::RB; j2++) lds[PL::B_OFF + PL::xi(s, j2, ka, t.col)] = r[j2];
+    for (int j2 = 0; j2 < PL::RB; j2++) lds[PL::B_OFF + PL::xi(s, j2, ka, t.ocol)] = r[j2];
   }
 }
 
-// P1': NTT over kA of the thread's (j2 = w, j1 = 2 sp + h) values (B),
+// P1': NTT over kA of the thread's (j2 = w, j1 = 2 sp + s) values (B),
 // canonical -> the ODD staging (odd rows 2 (j1 + RC j2 + RC RB j3) + 1).
 template <class PL>
-ZHD void p1b(const Tid& t, int32_t* __restrict__ lds, const MulK& k) {
+ZHD void p1b(const Tid<PL>& t, int32_t* __restrict__ lds, const MulK& k) {
   group_fence();
-  double x[PL::RA];
-#pragma unroll
-  for (int ka = 0; ka < PL::RA; ka++) x[ka] = (double)lds[PL::B_OFF + PL::xi(t.h, t.w, ka, t.col)];
-  dft<PL::LRA, false>(x, k);
   uint32_t* odd = reinterpret_cast<uint32_t*>(lds + PL::ODD_OFF);
+  if constexpr (!PL::SPLIT) {
+    double x[PL::RA];
 #pragma unroll
-  for (int j3 = 0; j3 < PL::RA; j3++) odd[PL::si(t.h, j3, t.w, t.col)] = canon(x[j3], k);
+    for (int ka = 0; ka < PL::RA; ka++) x[ka] = (double)lds[PL::B_OFF + PL::xi(t.s, t.w, ka, t.col)];
+    dft<PL::LRA, false>(x, k);
+#pragma unroll
+    for (int j3 = 0; j3 < PL::RA; j3++) odd[PL::si(t.s, j3, t.w, t.col)] = canon(x[j3], k);
+  } else {
+    // split as in p1, forward: j3 = 2m + hh
+    constexpr int H = PL::RA / 2;
+    const double hd = (double)t.hh, sgn = 1.0 - 2.0 * hd;
+    double x[H];
+#pragma unroll
+    for (int n = 0; n < H; n++)
+      x[n] = fmad((double)lds[PL::B_OFF + PL::xi(t.s, t.w, n + H, t.col)], sgn,
+                  (double)lds[PL::B_OFF + PL::xi(t.s, t.w, n, t.col)]);
+    split_input_twist<PL::LRA, false>(x, hd, k);
+    dft<PL::LRA - 1, false>(x, k);
+#pragma unroll
+    for (int m = 0; m < H; m++) odd[PL::si(t.s, 2 * m + t.hh, t.w, t.col)] = canon(x[m], k);
+  }
 }
 
 }  // namespace slice
