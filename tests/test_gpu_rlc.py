@@ -1,9 +1,9 @@
 """GPU parity of the random linear combinations (src/zoda/rlc.rs:32-66) and
 the commitment check they serve (src/benchmarks/utils.rs:221-272): the
 device-derived coefficients equal the oracle's SHA-256 derivation, the device
-row RLC equals the oracle's rlc_row, and on the 2048^2 square the RLC of every
-extended row equals the extension of the original rows' RLC (the two are
-F_p-linear and commute)."""
+row RLC equals the oracle's rlc_row, and on the 2048^2 and 4096^2 squares the
+RLC of every extended row equals the extension of the original rows' RLC (the
+two are F_p-linear and commute)."""
 import numpy as np
 import pytest
 
@@ -96,22 +96,32 @@ def test_rlc_rows_unaligned_square(zh, torch):
         assert np.array_equal(got[r], O.rlc_row(sq[r], c)), r
 
 
-def test_commutes_with_the_extension_2048(zh, torch):
-    # RLC(extended row i) = sum_j G[i][j] RLC(original row j) for every one of
-    # the 4096 rows of the 2048^2 square: each limb of the original rows' RLC,
-    # extended by the oracle (INTT-2048, pad, NTT-4096), must equal the RLC of
-    # the device-encoded rows.
-    side = 2048
+@pytest.mark.parametrize("side", [2048, 4096])
+def test_commutes_with_the_extension(zh, torch, side):
+    # RLC(extended row i) = sum_j G[i][j] RLC(original row j) for every row of
+    # the 2048^2 (4096 rows) and 4096^2 (8192 rows) squares: each limb of the
+    # original rows' RLC, extended by the oracle (INTT-s, pad, NTT-2s), must
+    # equal the RLC of the device-encoded rows.  A wrong value anywhere in the
+    # output changes its row's RLC, so this checks every value the encode
+    # writes.  Three encodes into a zeroed output: a fault that depends on
+    # timing (such as registers of a store overwritten before the store has
+    # read them) shows as a difference in any one of them.
     cols = 128 * side
     print(f"[case] rlc commute {side}x{cols}", flush=True)
     t_in = torch.empty((side, cols), dtype=torch.int32, device="cuda")
-    zh.fill_uniform(t_in, seed=2048)
-    t_out = zh.encode_tensor(t_in)
+    zh.fill_uniform(t_in, seed=side)
     coeffs = zh.rlc_coefficients("ab" * 32, cols)
-    ext = u32(zh.rlc_rows(t_out, coeffs))          # (4096, 4)
-    orig = u32(zh.rlc_rows(t_in, coeffs))          # (2048, 4)
-    for limb in range(4):
-        expect = O.encode(orig[:, limb].reshape(side, 1), threads=4).ravel()
-        assert np.array_equal(ext[:, limb], expect), limb
-    # and the systematic rows: the even extended rows are the original rows
-    assert np.array_equal(ext[0::2], orig)
+    orig = u32(zh.rlc_rows(t_in, coeffs))          # (side, 4)
+    expect = np.stack([O.encode(np.ascontiguousarray(orig[:, limb]).reshape(side, 1), threads=4).ravel()
+                       for limb in range(4)], axis=1)
+    t_out = torch.empty((2 * side, cols), dtype=torch.int32, device="cuda")
+    for rep in range(3):
+        t_out.zero_()
+        zh.encode_tensor(t_in, out=t_out)
+        ext = u32(zh.rlc_rows(t_out, coeffs))      # (2 side, 4)
+        bad = np.nonzero((ext != expect).any(axis=1))[0]
+        assert bad.size == 0, f"encode {rep}: {bad.size} rows differ, e.g. {bad[:8].tolist()}"
+        # and the systematic rows: the even extended rows are the original rows
+        assert np.array_equal(ext[0::2], orig)
+    del t_in, t_out
+    torch.cuda.empty_cache()

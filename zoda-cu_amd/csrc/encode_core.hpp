@@ -245,8 +245,20 @@ struct GBuf {
   __device__ U4 load4(uint32_t voff, uint32_t soff) const {
     return __builtin_bit_cast(U4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
   }
+  // A 16-byte store reads the upper half of its data registers after it
+  // issues, and on gfx950 the compiler's hazard model (which sees no hazard
+  // for a buffer store with a register offset) lets arithmetic overwrite those
+  // registers first: the side-4096 kernel, with stores followed within four
+  // instructions by FP64 writes of their registers, stored such values in
+  // place of data (bytes 8-15 of some lanes, a few hundred values in every
+  // 4096^2 encode; none in any check with one wait state after each store).
+  // Each store is therefore followed by two wait states and a wait for the
+  // export counter (which on the first GCN parts also counted store data not
+  // yet read from registers); the asm also keeps the compiler from moving
+  // arithmetic in between the stores of a group.
   __device__ void store4(const U4& v, uint32_t voff, uint32_t soff) const {
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, v), r, voff, soff, 2);
+    asm volatile("s_waitcnt expcnt(0)\n\ts_nop 1" ::: "memory");
   }
 #else
   char* b;

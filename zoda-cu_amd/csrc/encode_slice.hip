@@ -196,12 +196,16 @@ __device__ __forceinline__ void run_tiles(const uint32_t* in, uint32_t* out, uin
     slice::load_step<PL>(ib, c.o, c.wu, 1, cols * 4u, ld);
   }
   lds_barrier();  // the LDS tables and the first IN
-  uint32_t tp = 0;
   for (uint32_t it = 0; it <= mine; it++) {
     const bool front = it < mine, back = it > 0, next = it + 1 < mine, prev = it > 1;
     const uint32_t tf = front ? tile(it) : 0u;
     const uint32_t tb = back ? tile(it - 1) : 0u;
     const uint32_t tn = next ? tile(it + 1) : 0u;
+    // the tile whose odd rows the previous step left in the staging, computed
+    // afresh rather than carried: a value carried around the loop can settle
+    // in vector registers, and a descriptor built from it wraps its stores in
+    // a loop over lanes
+    const uint32_t tp = prev ? tile(it - 2) : 0u;
     const Bufs<PL> bf{tile_buf<PL>(in + (size_t)tf * PL::W, front), tile_buf<PL>(out + (size_t)tf * PL::W, front),
                       tile_buf<PL>(in + (size_t)tn * PL::W, next),  tile_buf<PL>(out + (size_t)tn * PL::W, next),
                       tile_buf<PL>(out + (size_t)tb * PL::W, back), tile_buf<PL>(out + (size_t)tp * PL::W, prev)};
@@ -216,10 +220,10 @@ __device__ __forceinline__ void run_tiles(const uint32_t* in, uint32_t* out, uin
     const Ctx<PL> ci{tt, slice::Offs<PL>(tt, cols), c.wu, lds, c.t1w, c.t2, c.t3, c.k};
     step<PL, EVM, 0, MF>(ci, c4, front, back, bf, ld, held);
     if (front) mid<PL, EVM, 0>(ci, c4, bf, held);
-    tp = tb;
   }
   // the odd rows of the last tile's last step
-  const BufT<PL> ob = tile_buf<PL>(out + (size_t)tp * PL::W, mine > 0);
+  const uint32_t tl = mine > 0 ? tile(mine - 1) : 0u;
+  const BufT<PL> ob = tile_buf<PL>(out + (size_t)tl * PL::W, mine > 0);
   slice::store_odd<PL>(c.t, ob, c.o, PL::STEPS - 1, cols * 4u, lds);
 }
 

@@ -262,6 +262,20 @@ ZHD void mul_mem_batch(const double (&x)[NV], const TW& tw, int32_t (&r)[NV], co
 
 // ---- the plan ------------------------------------------------------------------------
 
+// The wave index of thread tid, wave-uniform (a scalar register on the
+// device).  Row offsets built from it stay scalar, so the raw-buffer stores
+// take them as their uniform offset; from the plain shift the compiler cannot
+// tell that the offset is the same on every lane and wraps each store in a
+// loop over its distinct values.
+ZHD int wave_of(int tid) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_readfirstlane(tid >> 6);
+#else
+  return tid >> 6;
+#endif
+}
+
+
 struct Plan2048 {
   static constexpr int LRA = 4, LRB = 3, LRC = 4;
   static constexpr bool SPLIT = false;  // P1 / P1' DFTs whole in one thread
@@ -306,7 +320,7 @@ struct Plan2048 {
   // nB (resp. j2) = w; owner (P2, middle, P2') the same column and kA = 2 w + s;
   // 16-byte access lane: column quad q, row g.
   static ZHD void decode(int tid, int& w, int& s, int& hh, int& col, int& ocol, int& oka, int& q, int& g) {
-    w = tid >> 6;
+    w = wave_of(tid);
     s = (tid >> 5) & 1;
     hh = 0;
     col = tid & 31;
@@ -362,7 +376,7 @@ struct Plan4096 {
   // nB (resp. j2) = w; owner column = lane mod 16, kA = 4 w + lane / 16;
   // 16-byte access lane: column quad q (bits 0-1), row g (bits 2-5).
   static ZHD void decode(int tid, int& w, int& s, int& hh, int& col, int& ocol, int& oka, int& q, int& g) {
-    w = tid >> 6;
+    w = wave_of(tid);
     s = (tid >> 4) & 1;
     hh = (tid >> 5) & 1;
     col = tid & 15;
