@@ -191,17 +191,17 @@ struct SliceSim {
     if constexpr (SP + 1 < PL::STEPS) step<SP + 1>(b, front, back);
   }
 
-  // the middle passes; group KB copies step KB's even rows when its bit of
-  // EVEN_MID_MASK is set, as on the device
+  // the middle passes; group KB loads step KB's rows when its bit of
+  // EVEN_MID_MASK is set and group KB + 1 stores them as even rows, as on the
+  // device
   template <int KB>
-  void mid(int tid, const Bufs& b) {
+  void mid(int tid, const Bufs& b, U4 (&ev)[PL::QI]) {
     const Tid t(tid);
     const slice::Offs<PL> o(t, cols);
-    U4 ev[PL::QI];
     if constexpr (PL::even_in_mid(KB)) slice::load_step<PL>(b.ibf, o, t.w, KB, cols * 4, ev);
     slice::mid_group<PL, KB>(h(tid), k);
-    if constexpr (PL::even_in_mid(KB)) slice::store_even<PL>(b.obf, o, t.w, KB, cols * 4, ev);
-    if constexpr (KB + 1 < PL::RB) mid<KB + 1>(tid, b);
+    if constexpr (KB > 0 && PL::even_in_mid(KB - 1)) slice::store_even<PL>(b.obf, o, t.w, KB - 1, cols * 4, ev);
+    if constexpr (KB + 1 < PL::RB) mid<KB + 1>(tid, b, ev);
   }
 
   B gb(const uint32_t* base, uint32_t tile, bool live) const {
@@ -228,7 +228,10 @@ struct SliceSim {
                    gb(out, tb, back), gb(out, tp, prev)};
       step<0>(b, front, back);
       if (front)
-        for (int tid = 0; tid < PL::NT; tid++) mid<0>(tid, b);
+        for (int tid = 0; tid < PL::NT; tid++) {
+          U4 ev[PL::QI];
+          mid<0>(tid, b, ev);
+        }
       tp = tb;
     }
     for (int tid = 0; tid < PL::NT; tid++) {

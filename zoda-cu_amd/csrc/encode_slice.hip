@@ -145,24 +145,25 @@ __device__ __forceinline__ void step(const Ctx<PL>& c, uint32_t c4, bool front, 
   if constexpr (SP + 1 < PL::STEPS) step<PL, EVM, SP + 1, MF>(c, c4, front, back, b, ld, held);
 }
 
-// The middle passes of the front tile; group KB copies step KB's even rows
-// (read again, 16 bytes a lane) when bit KB of EVM is set, so that the CU
-// keeps memory requests in flight through the passes that have no memory
-// work of their own.
+// The middle passes of the front tile.  Group KB loads step KB's input rows
+// (16 bytes a lane) when bit KB of EVM is set, and group KB + 1 stores them as
+// the even output rows, so that the CU keeps memory requests in flight
+// through the passes that have no memory work of their own, and each load has
+// two groups' arithmetic to arrive before its data is needed (1 % over
+// storing in the same group).
 template <class PL, unsigned EVM, int KB>
 __device__ __forceinline__ void mid(const Ctx<PL>& c, uint32_t c4, const Bufs<PL>& b,
-                                    int32_t (&held)[PL::RB][PL::RC]) {
-  U4 ev[PL::QI];
-  constexpr bool COPY = (EVM >> KB) & 1u;
-  if constexpr (COPY) slice::load_step<PL>(b.ibf, c.o, c.wu, KB, c4, ev);
+                                    int32_t (&held)[PL::RB][PL::RC], U4 (&ev)[PL::QI]) {
+  static_assert((EVM & (EVM >> 1)) == 0 && !(EVM >> (PL::RB - 1)), "copies one group apart, none in the last");
+  constexpr bool LOAD = (EVM >> KB) & 1u;
+  constexpr bool STORE = KB > 0 && ((EVM >> (KB - 1)) & 1u);
+  if constexpr (LOAD) slice::load_step<PL>(b.ibf, c.o, c.wu, KB, c4, ev);
   slice::mid_group<PL, KB>(held, c.k);
-  if constexpr (COPY) {
+  if constexpr (STORE) {
     group_fence();
-#pragma unroll
-    for (int i = 0; i < PL::QI; i++)
-      slice::acc_store4(b.obf, ev[i], c.o.ev, slice::Offs<PL>::b_out(KB, c.wu), slice::Offs<PL>::o_out(KB, c.wu, i), c4);
+    slice::store_even<PL>(b.obf, c.o, c.wu, KB - 1, c4, ev);
   }
-  if constexpr (KB + 1 < PL::RB) mid<PL, EVM, KB + 1>(c, c4, b, held);
+  if constexpr (KB + 1 < PL::RB) mid<PL, EVM, KB + 1>(c, c4, b, held, ev);
 }
 
 // The waves of the upper half of the work-group run their steps memory first.
@@ -219,7 +220,10 @@ __device__ __forceinline__ void run_tiles(const uint32_t* in, uint32_t* out, uin
     const slice::Tid<PL> tt(ti);
     const Ctx<PL> ci{tt, slice::Offs<PL>(tt, cols), c.wu, lds, c.t1w, c.t2, c.t3, c.k};
     step<PL, EVM, 0, MF>(ci, c4, front, back, bf, ld, held);
-    if (front) mid<PL, EVM, 0>(ci, c4, bf, held);
+    if (front) {
+      U4 ev[PL::QI];
+      mid<PL, EVM, 0>(ci, c4, bf, held, ev);
+    }
   }
   // the odd rows of the last tile's last step
   const uint32_t tl = mine > 0 ? tile(mine - 1) : 0u;
