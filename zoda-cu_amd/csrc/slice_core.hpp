@@ -360,7 +360,12 @@ struct Plan4096 {
   static_assert(W * RA == NT, "owners (column, kA)");
   static constexpr int QI = 4;
   static_assert(QI * 16 * WAVES == SPS * RA * RB, "one 16-byte access per lane covers a step");
-  static constexpr unsigned EVEN_MID_MASK = 0x55u;
+  // Every step's even rows are written from the staged input (no second
+  // reads in the middle passes): at this side the extra reads cost more than
+  // the memory work they give the middle passes (4096^2: 11.66-11.69 ms with
+  // none against 11.98 with steps 0 and 4 copied and 12.33 with every other
+  // step, alternating runs on one box).
+  static constexpr unsigned EVEN_MID_MASK = 0x00u;
   static constexpr ZHD bool even_in_mid(int s) { return (EVEN_MID_MASK >> s) & 1u; }
   static_assert(EVEN_MID_MASK == 0 || RB == STEPS, "one step of even rows per middle group");
   static constexpr int XWORDS = SPS * RB * RA * W;
