@@ -102,6 +102,8 @@ def sim_lib():
         lib.zoda_sim_encode_slice.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p]
         lib.zoda_sim_encode_persistent.restype = ctypes.c_int
         lib.zoda_sim_encode_persistent.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p]
+        lib.zoda_sim_ntt_fp64.restype = ctypes.c_int
+        lib.zoda_sim_ntt_fp64.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
         _sim = lib
     return _sim
 

@@ -16,6 +16,7 @@
 #include "../../zoda-cu_amd/csrc/encode_tables.hpp"
 #include "../../zoda-cu_amd/csrc/decode_core.hpp"
 #include "../../zoda-cu_amd/csrc/slice_core.hpp"
+#include "../../zoda-cu_amd/csrc/ntt_fp64_core.hpp"
 
 #include <array>
 
@@ -267,6 +268,65 @@ extern "C" int zoda_sim_encode_slice(const uint32_t* in, uint32_t k, uint32_t co
   if (k == (uint32_t)slice::Plan2048::S) return sim_slice<slice::Plan2048>(in, cols, out);
   if (k == (uint32_t)slice::Plan4096::S) return sim_slice<slice::Plan4096>(in, cols, out);
   return -3;
+}
+
+// The FP64 poly transform (ntt_poly_fp64) run on the host: every tile's load,
+// passes and store for all 256 threads, phase by phase, with the kernel's LDS
+// layout (padded data, then the twiddle tables).
+template <int L, bool INV>
+static int sim_ntt_fp64_plan(uint32_t* values, uint32_t polys, uint32_t stride) {
+  using PL = nttf::Plan<L>;
+  std::vector<int32_t> lds(PL::LDS_WORDS, 0);
+  uint32_t* tw = reinterpret_cast<uint32_t*>(lds.data() + nttf::LDS_DATA);
+  std::vector<uint32_t> tab(std::max(1, PL::TW_WORDS));
+  nttf::build_ntt_fp64_tables<PL>(INV, tab.data());
+  for (int i = 0; i < PL::TW_WORDS; i++) tw[i] = tab[i];
+  const slice::MulK k = slice::make_mulk();
+  const uint32_t ntiles = (polys + PL::POLYS - 1) / PL::POLYS;
+  for (uint32_t t = 0; t < ntiles; t++) {
+    const uint32_t first = t * (uint32_t)PL::POLYS;
+    const nttf::TileSpan ts{std::min((uint32_t)PL::POLYS, polys - first), (size_t)first};
+    for (int tid = 0; tid < nttf::NT; tid++)
+      nttf::load_tile<PL>(tid, ts, lds.data(), [&](size_t poly, uint32_t j) -> U4 {
+        const uint32_t* q = values + poly * stride + j;
+        return U4{q[0], q[1], q[2], q[3]};
+      });
+    for (int tid = 0; tid < nttf::NT; tid++) nttf::pass<PL, 0, INV>(tid, lds.data(), tw, k);
+    if constexpr (PL::P > 1)
+      for (int tid = 0; tid < nttf::NT; tid++) nttf::pass<PL, 1, INV>(tid, lds.data(), tw, k);
+    if constexpr (PL::P > 2)
+      for (int tid = 0; tid < nttf::NT; tid++) nttf::pass<PL, 2, INV>(tid, lds.data(), tw, k);
+    for (int tid = 0; tid < nttf::NT; tid++)
+      nttf::store_tile<PL>(tid, ts, lds.data(), [&](size_t poly, uint32_t kq, const U4& v) {
+        uint32_t* q = values + poly * stride + kq;
+        q[0] = v.x;
+        q[1] = v.y;
+        q[2] = v.z;
+        q[3] = v.w;
+      });
+  }
+  return 0;
+}
+
+template <int L>
+static int sim_ntt_fp64_size(uint32_t* values, uint32_t log_n, uint32_t polys, uint32_t stride, int inverse) {
+  if constexpr (L <= 12) {
+    if (log_n == (uint32_t)L)
+      return inverse ? sim_ntt_fp64_plan<L, true>(values, polys, stride)
+                     : sim_ntt_fp64_plan<L, false>(values, polys, stride);
+    return sim_ntt_fp64_size<L + 1>(values, log_n, polys, stride, inverse);
+  } else {
+    return -1;
+  }
+}
+
+// In place on host memory, the layout of the drop-in transforms (poly c at
+// values[c * stride], n = 2^log_n of it transformed); log_n in 2..12, stride a
+// multiple of 4.
+extern "C" int zoda_sim_ntt_fp64(uint32_t* values, uint32_t log_n, uint32_t polys, uint32_t stride,
+                                 int inverse) {
+  if (log_n < 2 || log_n > 12 || stride % 4 != 0 || stride < (1u << log_n)) return -1;
+  return sim_ntt_fp64_size<2>(values, log_n, polys, stride, inverse);
 }
 
 // variant < 0: the single-square plan of the side; 0..3: the batched plans.

@@ -21,6 +21,9 @@ int zoda_sim_encode_slice(const uint32_t* in, uint32_t k, uint32_t cols, uint32_
 int zoda_sim_decode(const uint32_t* enc, uint32_t k, uint32_t cols, const uint32_t* known_rows, uint32_t nknown,
                     uint32_t* out);
 int zo_encode(const uint32_t* input, uint32_t k, uint32_t cols, uint32_t n, uint32_t* out, int layout, int threads);
+int zoda_sim_ntt_fp64(uint32_t* values, uint32_t log_n, uint32_t polys, uint32_t stride, int inverse);
+int zo_ntt_babybear(uint32_t* values, uint32_t n);
+int zo_intt_babybear(uint32_t* values, uint32_t n);
 }
 
 namespace {
@@ -156,7 +159,27 @@ void check_decoder() {
 
 }  // namespace
 
+// The FP64 poly transform for every size 2^2..2^12, both directions, a
+// ragged last tile and a stride past n, against the oracle's ntt / intt.
+void check_poly_transforms() {
+  for (uint32_t L = 2; L <= 12; L++) {
+    const uint32_t n = 1u << L, polys = 4096 / n + 1, stride = n + 4;
+    for (int inverse = 0; inverse < 2; inverse++) {
+      std::vector<uint32_t> v = square(polys, stride, false), want = v;
+      for (uint32_t c = 0; c < polys; c++)
+        (inverse ? zo_intt_babybear : zo_ntt_babybear)(want.data() + (size_t)c * stride, n);
+      if (zoda_sim_ntt_fp64(v.data(), L, polys, stride, inverse) != 0) {
+        printf("FAILED ntt_fp64 L=%u inverse=%d\n", L, inverse);
+        g_failures++;
+        continue;
+      }
+      compare(inverse ? "intt_fp64" : "ntt_fp64", L, polys, 0, inverse, v, want);
+    }
+  }
+}
+
 int main() {
+  check_poly_transforms();
   check_plans();
   check_uneven();
   check_persistent_and_slice();

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import ROOT, sim_encode
+from helpers import ROOT, sim_encode, sim_lib
 
 P = O.P
 
@@ -219,6 +219,28 @@ def test_persistent_kernel_emulation(k, cols):
     rng = np.random.default_rng(k)
     data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)
     assert np.array_equal(sim_encode_persistent(data), O.encode(data, threads=4))
+
+
+@pytest.mark.parametrize("log_n", list(range(2, 13)))
+@pytest.mark.parametrize("inverse", [0, 1])
+def test_fp64_poly_transform_emulation(log_n, inverse):
+    # The drop-in transform's FP64 digit-pass kernel (ntt_fp64_core.hpp), run
+    # on the host phase by phase, against the oracle's ntt / intt
+    # (src/ntt/cpu.rs): a ragged last tile, a stride past n, one poly.
+    n = 1 << log_n
+    rng = np.random.default_rng(log_n * 2 + inverse)
+    lib = sim_lib()
+    for polys, extra in ((3, 0), (4096 // n + 1, 4), (1, 12)):
+        stride = n + extra
+        v = rng.integers(0, O.P, size=polys * stride, dtype=np.uint32)
+        got = v.copy()
+        assert lib.zoda_sim_ntt_fp64(got.ctypes.data, log_n, polys, stride, inverse) == 0
+        assert np.array_equal(got, O.ntt_polys(v, n, stride, inverse=bool(inverse))), (polys, stride)
+    # extreme inputs: all p - 1 and alternating 0 / p - 1
+    for pattern in (np.full(n, O.P - 1, dtype=np.uint32), np.where(np.arange(n) % 2 == 1, O.P - 1, 0).astype(np.uint32)):
+        got = pattern.copy()
+        assert lib.zoda_sim_ntt_fp64(got.ctypes.data, log_n, 1, n, inverse) == 0
+        assert np.array_equal(got, O.ntt_polys(pattern, n, n, inverse=bool(inverse)))
 
 
 @pytest.mark.parametrize("tables", [0, 1])

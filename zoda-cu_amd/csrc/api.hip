@@ -146,8 +146,26 @@ int get_user_mont_tables(DeviceState* st, const uint32_t* roots, uint32_t log_n,
   ZODA_TRY(upload_mont(fwd, inv, &t));
   st->user_mont[log_n] = t;
   st->user_roots[log_n].assign(roots, roots + n);
+  st->user_standard.erase(log_n);  // decided again for the new roots
   *out = t;
   return 0;
+}
+
+// Called after get_user_mont_tables with the same roots: the answer is cached
+// beside the roots that table was built from (and dropped when they change).
+bool roots_are_standard(DeviceState* st, const uint32_t* roots, uint32_t log_n) {
+  std::lock_guard<std::recursive_mutex> lock(st->mu);
+  const size_t n = (size_t)1 << log_n;
+  const auto ur = st->user_roots.find(log_n);
+  const bool cached_roots = ur != st->user_roots.end() && std::memcmp(ur->second.data(), roots, n * 4) == 0;
+  if (cached_roots) {
+    const auto us = st->user_standard.find(log_n);
+    if (us != st->user_standard.end()) return us->second;
+  }
+  const std::vector<uint32_t> std_roots = standard_roots(log_n);
+  const bool same = std::memcmp(std_roots.data(), roots, n * 4) == 0;
+  if (cached_roots) st->user_standard[log_n] = same;
+  return same;
 }
 
 // ---- seeded uniform generator -------------------------------------------------
@@ -256,6 +274,7 @@ static int host_transform(uint32_t* values, const uint32_t* roots, uint32_t n, u
   const uint32_t log_n = log2_ceil(n);
   MontTables tabs;
   ZODA_TRY(get_user_mont_tables(st, roots, log_n, &tabs));
+  const bool standard = roots_are_standard(st, roots, log_n);
   constexpr int NBUF = 3;
   const size_t chunk_bytes = 128u << 20;
   const uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(polys, chunk_bytes / ((size_t)n * 4)));
@@ -271,7 +290,7 @@ static int host_transform(uint32_t* values, const uint32_t* roots, uint32_t n, u
     uint32_t* d = bufs[ci % NBUF];
     uint32_t* h = values + (size_t)p0 * stride;
     ZODA_HIP(hipMemcpy2DAsync(d, poly_bytes, h, pitch, poly_bytes, w, hipMemcpyHostToDevice, s));
-    ZODA_TRY(launch_ntt_poly(d, log_n, w, n, inverse, tabs, s));
+    ZODA_TRY(launch_ntt(st, d, log_n, w, n, inverse, tabs, standard, s));
     ZODA_HIP(hipMemcpy2DAsync(h, pitch, d, poly_bytes, poly_bytes, w, hipMemcpyDeviceToHost, s));
   }
   for (int b = 0; b < NBUF; b++) ZODA_HIP(hipStreamSynchronize(streams[b]));
@@ -297,8 +316,8 @@ extern "C" int zoda_hip_ntt_device(uint32_t* d_values, uint32_t n, uint32_t poly
   ZODA_TRY(current_state(&st));
   MontTables tabs;
   ZODA_TRY(get_mont_tables(st, log2_ceil(n), &tabs));
-  return launch_ntt_poly(d_values, log2_ceil(n), polys, stride, inverse ? 1 : 0, tabs,
-                         (hipStream_t)stream);
+  return launch_ntt(st, d_values, log2_ceil(n), polys, stride, inverse ? 1 : 0, tabs, true,
+                    (hipStream_t)stream);
 }
 
 static int check_encode_args(const void* in, uint32_t k, uint32_t cols, uint32_t n, const void* out,

@@ -110,6 +110,10 @@ struct DeviceState {
   void* slice_tables[2] = {nullptr, nullptr};  // sides 2048, 4096
   // This device's address of the RLC's static scratch array (rlc.hip).
   uint32_t* rlc_scratch = nullptr;
+  // Twiddle tables of the FP64 poly transforms (ntt_poly.hip), keyed by
+  // log n << 1 | inverse; whether a size's caller roots are the standard ones.
+  std::map<uint32_t, uint32_t*> ntt_fp64;
+  std::map<uint32_t, bool> user_standard;
 };
 
 // State of the calling thread's current device (created on first use).
@@ -122,9 +126,16 @@ int get_mont_tables(DeviceState* st, uint32_t log_n, MontTables* out);
 // Montgomery tables from caller roots (roots[i] = w^i, i < n).
 int get_user_mont_tables(DeviceState* st, const uint32_t* roots, uint32_t log_n, MontTables* out);
 
-// Batched poly-major transform on device memory (ntt_poly.hip).
+// Batched poly-major transform on device memory (ntt_poly.hip): radix-2
+// Montgomery stages with the tables' roots.
 int launch_ntt_poly(uint32_t* d_values, uint32_t log_n, uint32_t polys, uint32_t stride,
                     int inverse, const MontTables& tabs, hipStream_t stream);
+// The same transform, on FP64 digit passes when the roots are the standard
+// ones and the size, stride and alignment allow (else launch_ntt_poly).
+int launch_ntt(DeviceState* st, uint32_t* d_values, uint32_t log_n, uint32_t polys, uint32_t stride,
+               int inverse, const MontTables& tabs, bool standard_roots, hipStream_t stream);
+// Whether caller roots of size 2^log_n are the standard roots (cached).
+bool roots_are_standard(DeviceState* st, const uint32_t* roots, uint32_t log_n);
 
 inline uint32_t encode_table_key(int L, int r1, int r2, int rd) {
   return (uint32_t)L | ((uint32_t)r1 << 8) | ((uint32_t)r2 << 16) | ((uint32_t)rd << 24);

@@ -232,8 +232,8 @@ static int encode_general(DeviceState* st, const uint32_t* d_in, uint32_t k, uin
   MontTables tk, tkn;
   int rc = get_mont_tables(st, log_nk, &tk);
   if (!rc) rc = get_mont_tables(st, log_nkn, &tkn);
-  if (!rc) rc = launch_ntt_poly(tmp, log_nk, cols, n_kn, 1, tk, stream);
-  if (!rc) rc = launch_ntt_poly(tmp, log_nkn, cols, n_kn, 0, tkn, stream);
+  if (!rc) rc = launch_ntt(st, tmp, log_nk, cols, n_kn, 1, tk, true, stream);
+  if (!rc) rc = launch_ntt(st, tmp, log_nkn, cols, n_kn, 0, tkn, true, stream);
   if (!rc && layout == ZODA_LAYOUT_ROW) {
     hipLaunchKernelGGL(scatter_rows, dim3(blocks), dim3(256), 0, stream, tmp, cols, n_kn, d_out);
     hipError_t e = hipGetLastError();

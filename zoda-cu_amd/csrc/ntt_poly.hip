@@ -7,8 +7,10 @@
 // arithmetic of the reference's CPU ntt (src/ntt/cpu.rs:1-32) so any roots
 // the caller passes give the reference's answer.  Sizes up to 2^12 run
 // LDS-resident, one launch for all stages; larger sizes fall back to one
-// launch per stage on device memory.
+// launch per stage on device memory.  With the standard roots, sizes 2^2 to
+// 2^12 run the FP64 digit-pass kernel instead (ntt_fp64_core.hpp).
 #include "common.hpp"
+#include "ntt_fp64_core.hpp"
 
 namespace zoda {
 
@@ -97,6 +99,112 @@ __global__ void scale_global(uint32_t* __restrict__ data, uint32_t log_n, uint32
     uint32_t* v = data + (t >> log_n) * stride + (t & (n - 1));
     *v = mont_mul(*v, scale_mont);
   }
+}
+
+// ---- FP64 digit passes (standard roots, n = 2^2..2^12) ---------------------------
+
+// A 16-byte store followed by two wait states and a wait for the export
+// counter: on gfx950 a 16-byte store reads the upper half of its data
+// registers after it issues, and arithmetic that overwrites them a few
+// instructions later can win the race (see GBuf::store4).
+__device__ __forceinline__ void store16(uint32_t* p, const U4& v) {
+  using V4 = __attribute__((ext_vector_type(4))) unsigned int;
+  *reinterpret_cast<V4*>(p) = __builtin_bit_cast(V4, v);
+  asm volatile("s_waitcnt expcnt(0)\n\ts_nop 1" ::: "memory");
+}
+
+// Persistent: each work-group copies the plan's twiddle tables into LDS once,
+// then walks tiles of 4096 values (4096 / n polys): load, P passes, store.
+template <int L, bool INV>
+__global__ __launch_bounds__(nttf::NT) void ntt_poly_fp64(uint32_t* __restrict__ data, uint32_t polys,
+                                                          uint32_t stride, const uint32_t* __restrict__ twg) {
+  using PL = nttf::Plan<L>;
+  __shared__ int32_t lds[PL::LDS_WORDS];
+  uint32_t* tw = reinterpret_cast<uint32_t*>(lds + nttf::LDS_DATA);
+  const int tid = (int)threadIdx.x;
+  for (int i = tid; i < PL::TW_WORDS; i += nttf::NT) tw[i] = twg[i];
+  const slice::MulK k = slice::make_mulk();
+  const uint32_t ntiles = (polys + PL::POLYS - 1) / PL::POLYS;
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint32_t first = t * (uint32_t)PL::POLYS;
+    const nttf::TileSpan ts{min((uint32_t)PL::POLYS, polys - first), (size_t)first};
+    __syncthreads();  // the tables, or the previous tile's last reads of LDS
+    nttf::load_tile<PL>(tid, ts, lds, [&](size_t poly, uint32_t j) -> U4 {
+      return *reinterpret_cast<const U4*>(data + poly * stride + j);
+    });
+    __syncthreads();
+    nttf::pass<PL, 0, INV>(tid, lds, tw, k);
+    __syncthreads();
+    if constexpr (PL::P > 1) {
+      nttf::pass<PL, 1, INV>(tid, lds, tw, k);
+      __syncthreads();
+    }
+    if constexpr (PL::P > 2) {
+      nttf::pass<PL, 2, INV>(tid, lds, tw, k);
+      __syncthreads();
+    }
+    nttf::store_tile<PL>(tid, ts, lds, [&](size_t poly, uint32_t kq, const U4& v) {
+      store16(data + poly * stride + kq, v);
+    });
+  }
+}
+
+template <int L, bool INV>
+static int launch_fp64_plan(DeviceState* st, uint32_t* d_values, uint32_t polys, uint32_t stride,
+                            hipStream_t stream) {
+  using PL = nttf::Plan<L>;
+  const uint32_t key = ((uint32_t)L << 1) | (INV ? 1u : 0u);
+  const uint32_t* twg = nullptr;
+  {
+    std::lock_guard<std::recursive_mutex> lock(st->mu);
+    auto it = st->ntt_fp64.find(key);
+    if (it == st->ntt_fp64.end()) {
+      std::vector<uint32_t> host(std::max(1, PL::TW_WORDS));
+      nttf::build_ntt_fp64_tables<PL>(INV, host.data());
+      void* d = nullptr;
+      if (const hipError_t ea = hipMalloc(&d, host.size() * 4)) return alloc_fail(ea, "FP64 NTT tables");
+      const hipError_t e = hipMemcpy(d, host.data(), host.size() * 4, hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        (void)hipFree(d);
+        return fail((int)e, "copying FP64 NTT tables failed: %s", hipGetErrorString(e));
+      }
+      it = st->ntt_fp64.emplace(key, static_cast<uint32_t*>(d)).first;
+    }
+    twg = it->second;
+  }
+  const void* fn = (const void*)ntt_poly_fp64<L, INV>;
+  uint32_t grid = 0;
+  ZODA_TRY(kernel_setup(st, fn, nttf::NT, 0, true, &grid));
+  const uint32_t ntiles = (polys + PL::POLYS - 1) / PL::POLYS;
+  if (grid > ntiles) grid = ntiles;
+  void* args[] = {(void*)&d_values, (void*)&polys, (void*)&stride, (void*)&twg};
+  ZODA_HIP(hipLaunchKernel(fn, dim3(grid), dim3(nttf::NT), args, 0, stream));
+  return 0;
+}
+
+template <int L>
+static int launch_fp64_size(DeviceState* st, uint32_t* d, uint32_t log_n, uint32_t polys, uint32_t stride,
+                            int inverse, hipStream_t stream) {
+  if constexpr (L <= 12) {
+    if (log_n == (uint32_t)L)
+      return inverse ? launch_fp64_plan<L, true>(st, d, polys, stride, stream)
+                     : launch_fp64_plan<L, false>(st, d, polys, stride, stream);
+    return launch_fp64_size<L + 1>(st, d, log_n, polys, stride, inverse, stream);
+  } else {
+    return fail(ZODA_EINVAL, "no FP64 plan for 2^%u", log_n);
+  }
+}
+
+bool ntt_fp64_fits(const uint32_t* d_values, uint32_t log_n, uint32_t stride) {
+  return log_n >= 2 && log_n <= 12 && stride % 4 == 0 && ((uintptr_t)d_values & 15) == 0;
+}
+
+int launch_ntt(DeviceState* st, uint32_t* d_values, uint32_t log_n, uint32_t polys, uint32_t stride,
+               int inverse, const MontTables& tabs, bool standard_roots, hipStream_t stream) {
+  if (polys == 0) return 0;
+  if (standard_roots && ntt_fp64_fits(d_values, log_n, stride))
+    return launch_fp64_size<2>(st, d_values, log_n, polys, stride, inverse, stream);
+  return launch_ntt_poly(d_values, log_n, polys, stride, inverse, tabs, stream);
 }
 
 static unsigned grid_for(size_t work) {
