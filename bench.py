@@ -441,6 +441,51 @@ def dropin_leg(zoda_hip, t_in, t_out, reps: int = 5):
         sq.free()
 
 
+def dropin_device_leg(zoda_hip, torch, t_in, t_out, steps: int = 5):
+    """The same window on device memory: zoda_hip_ntt_device inverse (n_k,
+    stride n_kn) then forward (n_kn, stride n_kn) over the column-major square
+    held in HBM -- the drop-in transform kernel itself, without PCIe.  HIP
+    events on the launch stream, average of `steps` after one warm-up whose
+    result is checked against the device encode.  Algorithmic bytes: 8 per
+    transformed value, 8 (n_k + n_kn) cols per window."""
+    import numpy as np
+    side, cols = t_in.shape
+    nkn = 2 * side
+    sq = torch.zeros((cols, nkn), dtype=torch.int32, device=t_in.device)
+    sq[:, :side] = t_in.t()
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+
+    def xform():
+        zoda_hip.ntt_device(sq.data_ptr(), side, cols, nkn, True, sptr)
+        zoda_hip.ntt_device(sq.data_ptr(), nkn, cols, nkn, False, sptr)
+
+    xform()
+    torch.cuda.synchronize()
+    pick = [0, 1, cols // 3, cols - 1]
+    ok = all(np.array_equal(sq[c].cpu().numpy(), t_out[:, c].cpu().numpy()) for c in pick)
+    st = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    en = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    for i in range(steps):  # in place: later runs transform the previous result (same work)
+        st[i].record(stream)
+        xform()
+        en[i].record(stream)
+    torch.cuda.synchronize()
+    ms = sorted(a.elapsed_time(b) for a, b in zip(st, en))
+    avg = sum(ms) / len(ms)
+    del sq
+    torch.cuda.empty_cache()
+    return {
+        "window": f"zoda_hip_ntt_device inverse (n {side}, stride {nkn}) + forward (n {nkn}, stride {nkn}) on the "
+                  f"column-major {cols} x {nkn} square in HBM; HIP events, average of {steps}",
+        "ms_avg": round(avg, 4),
+        "ms_p50": round(ms[len(ms) // 2], 4),
+        "gbps": round(side * side * SHARE_BYTES * 8 / 1e9 / (avg / 1e3), 2),
+        "roofline": roofline_of(8 * (side + nkn) * cols, avg),
+        "matches_device_encode": ok,
+    }
+
+
 def column_major_leg(zoda_hip, torch, t_in, steps: int):
     """The device-resident encode in the reference's output layout
     (ZODA_LAYOUT_COL: out[c * n_kn + r], src/ntt/cuda.rs:31-67)."""
@@ -688,6 +733,7 @@ def main():
             result["gpu_xform_dropin"] = dropin_leg(zoda_hip, t_in, t_out)
         except Exception as ex:  # report, do not fail the device-resident line
             result["gpu_xform_dropin"] = {"error": str(ex)}
+        result["gpu_xform_device"] = dropin_device_leg(zoda_hip, torch, t_in, t_out)
     if not args.no_rlc:
         result["rlc"] = rlc_leg(zoda_hip, torch, t_out)
     if not args.no_decode:

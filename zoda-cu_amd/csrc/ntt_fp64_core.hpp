@@ -113,21 +113,28 @@ struct TileSpan {
 };
 
 // Phase 1: the tile's values into LDS (16-byte loads, four values a lane).
+// Chunk q of thread tid is the four values at 4 (tid + q NT) of the tile.
+template <class PL, class LoadF>
+ZHD U4 tile_chunk(int tid, int q, const TileSpan& ts, const LoadF& load4) {
+  const int v = 4 * (tid + q * NT), p = v >> PL::L, j = v & (PL::N - 1);
+  U4 x{0, 0, 0, 0};
+  if ((uint32_t)p < ts.live) x = load4(ts.poly0 + (size_t)p, (uint32_t)j);
+  return x;
+}
+
+ZHD void put_chunk(int tid, int q, int32_t* __restrict__ lds, const U4& x) {
+  int32_t* d = lds + phys(4 * (tid + q * NT));
+  d[0] = (int32_t)x.x;
+  d[1] = (int32_t)x.y;
+  d[2] = (int32_t)x.z;
+  d[3] = (int32_t)x.w;
+}
+
 template <class PL, class LoadF>
 ZHD void load_tile(int tid, const TileSpan& ts, int32_t* __restrict__ lds, const LoadF& load4) {
   constexpr int CHUNKS = TILE / 4 / NT;
 #pragma unroll
-  for (int q = 0; q < CHUNKS; q++) {
-    const int c = tid + q * NT;
-    const int v = 4 * c, p = v >> PL::L, j = v & (PL::N - 1);
-    U4 x{0, 0, 0, 0};
-    if ((uint32_t)p < ts.live) x = load4(ts.poly0 + (size_t)p, (uint32_t)j);
-    int32_t* d = lds + phys(v);
-    d[0] = (int32_t)x.x;
-    d[1] = (int32_t)x.y;
-    d[2] = (int32_t)x.z;
-    d[3] = (int32_t)x.w;
-  }
+  for (int q = 0; q < CHUNKS; q++) put_chunk(tid, q, lds, tile_chunk<PL>(tid, q, ts, load4));
 }
 
 // Pass I: the radix-2^R DFT over digit I of every group, then the twiddle
