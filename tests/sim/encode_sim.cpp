@@ -17,6 +17,7 @@
 #include "../../zoda-cu_amd/csrc/decode_core.hpp"
 #include "../../zoda-cu_amd/csrc/slice_core.hpp"
 #include "../../zoda-cu_amd/csrc/ntt_fp64_core.hpp"
+#include "../../zoda-cu_amd/csrc/encode_cols_core.hpp"
 
 #include <array>
 
@@ -327,6 +328,73 @@ extern "C" int zoda_sim_ntt_fp64(uint32_t* values, uint32_t log_n, uint32_t poly
                                  int inverse) {
   if (log_n < 2 || log_n > 12 || stride % 4 != 0 || stride < (1u << log_n)) return -1;
   return sim_ntt_fp64_size<2>(values, log_n, polys, stride, inverse);
+}
+
+// ---- column-major encode (encode_cols_core.hpp), phase by phase ----------------
+
+template <int L>
+static int sim_encode_cols_plan(const uint32_t* in, uint32_t cols, uint32_t* out) {
+  using PL = ecol::Plan<L>;
+  std::vector<int32_t> lds(PL::LDS_WORDS, 0);
+  uint32_t* tw = reinterpret_cast<uint32_t*>(lds.data() + nttf::LDS_DATA);
+  std::vector<uint32_t> tab(PL::TW_WORDS);
+  ecol::build_col_tables<PL>(tab.data());
+  for (int i = 0; i < PL::TW_WORDS; i++) tw[i] = tab[i];
+  const slice::MulK k = slice::make_mulk();
+  // the transpose the device runs first: t[c][r] = in[r][c]
+  std::vector<uint32_t> t((size_t)cols * PL::S);
+  for (uint32_t r = 0; r < (uint32_t)PL::S; r++)
+    for (uint32_t c = 0; c < cols; c++) t[(size_t)c * PL::S + r] = in[(size_t)r * cols + c];
+  const uint32_t ntiles = (cols + PL::COLS - 1) / PL::COLS;
+  std::vector<ecol::U2> held((size_t)nttf::NT * ecol::PAIRS);
+  for (uint32_t tile = 0; tile < ntiles; tile++) {
+    const size_t col0 = (size_t)tile * PL::COLS;
+    const uint32_t live = std::min((uint32_t)PL::COLS, cols - (uint32_t)col0);
+    for (int tid = 0; tid < nttf::NT; tid++)
+      for (int q = 0; q < ecol::PAIRS; q++) {
+        const ecol::U2 x = ecol::tile_pair<PL>(tid, q, live, col0, [&](size_t c, uint32_t j) -> ecol::U2 {
+          const uint32_t* p = t.data() + c * PL::S + j;
+          return ecol::U2{p[0], p[1]};
+        });
+        held[(size_t)tid * ecol::PAIRS + q] = x;
+        ecol::put_pair(tid, q, lds.data(), x);
+      }
+    ecol::all_passes<PL>(
+        [&](auto pass) {
+          for (int tid = 0; tid < nttf::NT; tid++) pass(tid, lds.data(), tw, k);
+        },
+        [] {});
+    for (int tid = 0; tid < nttf::NT; tid++) {
+      auto x = [&](int q) { return held[(size_t)tid * ecol::PAIRS + q]; };
+      ecol::store_pairs<PL>(tid, live, col0, lds.data(), x, [&](size_t c, uint32_t r, const U4& v) {
+        uint32_t* p = out + c * 2 * PL::S + r;
+        p[0] = v.x;
+        p[1] = v.y;
+        p[2] = v.z;
+        p[3] = v.w;
+      });
+    }
+  }
+  return 0;
+}
+
+template <int L>
+static int sim_encode_cols_size(const uint32_t* in, uint32_t log_k, uint32_t cols, uint32_t* out) {
+  if constexpr (L <= 12) {
+    if (log_k == (uint32_t)L) return sim_encode_cols_plan<L>(in, cols, out);
+    return sim_encode_cols_size<L + 1>(in, log_k, cols, out);
+  } else {
+    return -1;
+  }
+}
+
+// The column-major encode of a whole k x cols square (k = 2^log_k, 5..12):
+// out[c * 2k + r], as the device's transpose + column kernel compute it.
+extern "C" int zoda_sim_encode_cols(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out) {
+  uint32_t L = 0;
+  while ((1u << L) < k) L++;
+  if ((1u << L) != k || L < 5 || L > 12 || cols == 0) return -1;
+  return sim_encode_cols_size<5>(in, L, cols, out);
 }
 
 // variant < 0: the single-square plan of the side; 0..3: the batched plans.

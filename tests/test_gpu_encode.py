@@ -172,6 +172,27 @@ def test_slice_kernel_tile_counts_4096(zh, torch, tiles):
     assert np.array_equal(gpu_encode(zh, torch, data), O.encode(data, threads=16))
 
 
+@pytest.mark.parametrize("side,cols", [(2048, 64), (2048, 64 * 1000), (2048, 128 * 2048),
+                                       (4096, 64), (4096, 64 * 300)])
+def test_column_kernel_matches_row_layout(zh, torch, side, cols):
+    # whole squares with cols a multiple of 64 go into the column-major layout
+    # by transpose + column kernel: fewer tiles than work-groups, several per
+    # work-group, and the whole 2048^2 square of 512-byte shares; every value
+    # against the row-layout encode of the same data, sampled columns against
+    # the oracle
+    t = torch.empty((side, cols), dtype=torch.int32, device="cuda")
+    zh.fill_uniform(t, seed=side + cols)
+    print_case("cols-kernel", side=side, cols=cols)
+    col = zh.encode_tensor(t, layout=1)
+    row = zh.encode_tensor(t, layout=0)
+    torch.cuda.synchronize()
+    assert tuple(col.shape) == (cols, 2 * side)
+    assert torch.equal(col.t().contiguous(), row)
+    idx = sorted({0, 1, cols // 3, cols - 1})
+    data = to_host(t[:, idx].contiguous())
+    assert np.array_equal(to_host(col[idx, :].t().contiguous()), O.encode(data, threads=4))
+
+
 def test_column_major_slab_2048(zh, torch):
     # the exact slab zoda_hip_encode_host encodes at 2048^2 (4096 columns of
     # 2048 rows, whole tiles, column-major output), every value checked

@@ -162,6 +162,13 @@ bool slice_fits(uint32_t k, uint32_t cols, int layout);
 int launch_slice(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols, uint32_t* d_out,
                  hipStream_t stream, bool dry);
 
+// Column-major encode of a whole square by transpose + column kernel
+// (encode_cols.hip): cols_fits says whether it takes the shape on `stream`
+// (not while the stream is being captured: it allocates its scratch).
+bool cols_fits(const uint32_t* d_in, uint32_t k, uint32_t cols, const uint32_t* d_out, hipStream_t stream);
+int launch_cols(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32_t cols, uint32_t* d_out,
+                hipStream_t stream);
+
 // Before tuning, squares of side >= 2^DIRECT_MIN_LOG run as their own fused
 // launches in batched and served work (their kernel fills the device), next
 // to the persistent work queue of the smaller squares; the autotuner decides

@@ -163,6 +163,9 @@ static int launch_plan(DeviceState* st, const uint32_t* d_in, uint32_t k, uint32
   const size_t lds_bytes = (size_t)PL::LDS_WORDS * sizeof(int32_t);
   const bool full = cols % PL::W == 0 && k == (uint32_t)PL::S;
   const bool row = layout == ZODA_LAYOUT_ROW;
+  // whole squares into the column-major layout: transpose + column kernel
+  if (!row && !dry && k == (uint32_t)PL::S && cols_fits(d_in, k, cols, d_out, stream))
+    return launch_cols(st, d_in, k, cols, d_out, stream);
   // sides 2048 and 4096: the slice-pipelined kernel where it fits
   if constexpr (PL::L == 11 || PL::L == 12) {
     if (k == (uint32_t)PL::S && slice_fits(k, cols, layout))

@@ -282,6 +282,21 @@ struct GBuf {
 #endif
 };
 
+#if defined(__HIPCC__)
+// A 16-byte global store followed by the wait states of GBuf::store4 (the
+// same gfx950 store-data hazard).  NT selects the non-temporal cache policy,
+// for output that no kernel reads back.
+template <bool NT = false>
+__device__ __forceinline__ void store16(uint32_t* p, const U4& v) {
+  using V4 = __attribute__((ext_vector_type(4))) unsigned int;
+  if constexpr (NT)
+    __builtin_nontemporal_store(__builtin_bit_cast(V4, v), reinterpret_cast<V4*>(p));
+  else
+    *reinterpret_cast<V4*>(p) = __builtin_bit_cast(V4, v);
+  asm volatile("s_waitcnt expcnt(0)\n\ts_nop 1" ::: "memory");
+}
+#endif
+
 // One pass for thread `tid` of the work-group encoding tile `tile`.
 //   LO, R    the digit this pass transforms
 //   KIND     PASS_INTT / PASS_MID / PASS_NTT

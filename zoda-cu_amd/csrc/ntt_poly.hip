@@ -103,16 +103,6 @@ __global__ void scale_global(uint32_t* __restrict__ data, uint32_t log_n, uint32
 
 // ---- FP64 digit passes (standard roots, n = 2^2..2^12) ---------------------------
 
-// A 16-byte store followed by two wait states and a wait for the export
-// counter: on gfx950 a 16-byte store reads the upper half of its data
-// registers after it issues, and arithmetic that overwrites them a few
-// instructions later can win the race (see GBuf::store4).
-__device__ __forceinline__ void store16(uint32_t* p, const U4& v) {
-  using V4 = __attribute__((ext_vector_type(4))) unsigned int;
-  *reinterpret_cast<V4*>(p) = __builtin_bit_cast(V4, v);
-  asm volatile("s_waitcnt expcnt(0)\n\ts_nop 1" ::: "memory");
-}
-
 // Persistent: each work-group copies the plan's twiddle tables into LDS once,
 // then walks tiles of 4096 values (4096 / n polys): load, P passes, store.
 // From n = 512 the next tile's values are loaded into registers while the
