@@ -105,7 +105,7 @@ def sim_lib():
         lib.zoda_sim_ntt_fp64.restype = ctypes.c_int
         lib.zoda_sim_ntt_fp64.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
         lib.zoda_sim_encode_cols.restype = ctypes.c_int
-        lib.zoda_sim_encode_cols.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p]
+        lib.zoda_sim_encode_cols.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_uint32]
         _sim = lib
     return _sim
 
@@ -131,13 +131,14 @@ def sim_encode_slice(data: np.ndarray) -> np.ndarray:
     return out
 
 
-def sim_encode_cols(data: np.ndarray) -> np.ndarray:
-    """The column-major encode (transpose + column kernel) run on the host:
-    the (cols, 2k) column-major extension of a k x cols square."""
+def sim_encode_cols(data: np.ndarray, chunk: int | None = None) -> np.ndarray:
+    """The column-major encode (chunked transposes + column kernel) run on the
+    host: the (cols, 2k) column-major extension of a k x cols square, cols a
+    multiple of 64, chunks of `chunk` columns (default: one chunk)."""
     k, cols = data.shape
     out = np.zeros((cols, 2 * k), dtype=np.uint32)
     d = np.ascontiguousarray(data, dtype=np.uint32)
-    rc = sim_lib().zoda_sim_encode_cols(d.ctypes.data, k, cols, out.ctypes.data)
+    rc = sim_lib().zoda_sim_encode_cols(d.ctypes.data, k, cols, out.ctypes.data, chunk or cols)
     if rc != 0:
         raise ValueError(f"no column plan for k={k} (status {rc})")
     return out

@@ -332,69 +332,107 @@ extern "C" int zoda_sim_ntt_fp64(uint32_t* values, uint32_t log_n, uint32_t poly
 
 // ---- column-major encode (encode_cols_core.hpp), phase by phase ----------------
 
+// The device's schedule (encode_cols.hip): chunks of `chunk` columns, launch i
+// transposing chunk i block by block into one buffer and encoding chunk
+// i - 1 tile by tile from the other.
 template <int L>
-static int sim_encode_cols_plan(const uint32_t* in, uint32_t cols, uint32_t* out) {
+static int sim_encode_cols_plan(const uint32_t* in, uint32_t cols, uint32_t* out, uint32_t chunk) {
   using PL = ecol::Plan<L>;
   std::vector<int32_t> lds(PL::LDS_WORDS, 0);
   uint32_t* tw = reinterpret_cast<uint32_t*>(lds.data() + nttf::LDS_DATA);
+  uint32_t* sh = reinterpret_cast<uint32_t*>(lds.data());
   std::vector<uint32_t> tab(PL::TW_WORDS);
   ecol::build_col_tables<PL>(tab.data());
   for (int i = 0; i < PL::TW_WORDS; i++) tw[i] = tab[i];
   const slice::MulK k = slice::make_mulk();
-  // the transpose the device runs first: t[c][r] = in[r][c]
-  std::vector<uint32_t> t((size_t)cols * PL::S);
-  for (uint32_t r = 0; r < (uint32_t)PL::S; r++)
-    for (uint32_t c = 0; c < cols; c++) t[(size_t)c * PL::S + r] = in[(size_t)r * cols + c];
-  const uint32_t ntiles = (cols + PL::COLS - 1) / PL::COLS;
+  const uint32_t nchunks = (cols + chunk - 1) / chunk;
+  std::vector<uint32_t> buf[2] = {std::vector<uint32_t>((size_t)PL::S * chunk),
+                                  std::vector<uint32_t>((size_t)PL::S * chunk)};
   std::vector<ecol::U2> held((size_t)nttf::NT * ecol::PAIRS);
-  for (uint32_t tile = 0; tile < ntiles; tile++) {
-    const size_t col0 = (size_t)tile * PL::COLS;
-    const uint32_t live = std::min((uint32_t)PL::COLS, cols - (uint32_t)col0);
-    for (int tid = 0; tid < nttf::NT; tid++)
-      for (int q = 0; q < ecol::PAIRS; q++) {
-        const ecol::U2 x = ecol::tile_pair<PL>(tid, q, live, col0, [&](size_t c, uint32_t j) -> ecol::U2 {
-          const uint32_t* p = t.data() + c * PL::S + j;
-          return ecol::U2{p[0], p[1]};
-        });
-        held[(size_t)tid * ecol::PAIRS + q] = x;
-        ecol::put_pair(tid, q, lds.data(), x);
+  for (uint32_t i = 0; i <= nchunks; i++) {
+    // the transpose of chunk i
+    if (i < nchunks) {
+      const uint32_t c0 = i * chunk, tcols = std::min(chunk, cols - c0);
+      if (tcols % ecol::TB) return -1;
+      const uint32_t nblocks = (PL::S / ecol::TB) * (tcols / ecol::TB);
+      std::vector<uint32_t>& t = buf[i % 2];
+      for (uint32_t bi = 0; bi < nblocks; bi++) {
+        const ecol::TBlock tb = ecol::tblock(bi, PL::S);
+        const uint32_t* src = in + (size_t)tb.r0 * cols + c0 + tb.c0;
+        for (int tid = 0; tid < nttf::NT; tid++)
+          for (int q = 0; q < ecol::TB_RUNS; q++) {
+            const uint32_t* p = src + ecol::tb_in_offset(tid, q, cols);
+            ecol::tb_put(tid, q, sh, U4{p[0], p[1], p[2], p[3]});
+          }
+        uint32_t* dst = t.data() + (size_t)tb.c0 * PL::S + tb.r0;
+        for (int tid = 0; tid < nttf::NT; tid++)
+          for (int q = 0; q < ecol::TB_RUNS; q++) {
+            const U4 v = ecol::tb_get(tid, q, sh);
+            uint32_t* p = dst + ecol::tb_out_offset(tid, q, PL::S);
+            p[0] = v.x;
+            p[1] = v.y;
+            p[2] = v.z;
+            p[3] = v.w;
+          }
       }
-    ecol::all_passes<PL>(
-        [&](auto pass) {
-          for (int tid = 0; tid < nttf::NT; tid++) pass(tid, lds.data(), tw, k);
-        },
-        [] {});
-    for (int tid = 0; tid < nttf::NT; tid++) {
-      auto x = [&](int q) { return held[(size_t)tid * ecol::PAIRS + q]; };
-      ecol::store_pairs<PL>(tid, live, col0, lds.data(), x, [&](size_t c, uint32_t r, const U4& v) {
-        uint32_t* p = out + c * 2 * PL::S + r;
-        p[0] = v.x;
-        p[1] = v.y;
-        p[2] = v.z;
-        p[3] = v.w;
-      });
+    }
+    // the column tiles of chunk i - 1
+    if (i > 0) {
+      const uint32_t c0 = (i - 1) * chunk, ccols = std::min(chunk, cols - c0);
+      const std::vector<uint32_t>& t = buf[(i - 1) % 2];
+      uint32_t* o = out + (size_t)c0 * 2 * PL::S;
+      const uint32_t ntiles = (ccols + PL::COLS - 1) / PL::COLS;
+      for (uint32_t tile = 0; tile < ntiles; tile++) {
+        const size_t col0 = (size_t)tile * PL::COLS;
+        const uint32_t live = std::min((uint32_t)PL::COLS, ccols - (uint32_t)col0);
+        for (int tid = 0; tid < nttf::NT; tid++)
+          for (int q = 0; q < ecol::PAIRS; q++) {
+            const ecol::U2 x = ecol::tile_pair<PL>(tid, q, live, col0, [&](size_t c, uint32_t j) -> ecol::U2 {
+              const uint32_t* p = t.data() + c * PL::S + j;
+              return ecol::U2{p[0], p[1]};
+            });
+            held[(size_t)tid * ecol::PAIRS + q] = x;
+            ecol::put_pair(tid, q, lds.data(), x);
+          }
+        ecol::all_passes<PL>(
+            [&](auto pass) {
+              for (int tid = 0; tid < nttf::NT; tid++) pass(tid, lds.data(), tw, k);
+            },
+            [] {});
+        for (int tid = 0; tid < nttf::NT; tid++) {
+          auto x = [&](int q) { return held[(size_t)tid * ecol::PAIRS + q]; };
+          ecol::store_pairs<PL>(tid, live, col0, lds.data(), x, [&](int, size_t c, uint32_t r, const U4& v) {
+            uint32_t* p = o + c * 2 * PL::S + r;
+            p[0] = v.x;
+            p[1] = v.y;
+            p[2] = v.z;
+            p[3] = v.w;
+          });
+        }
+      }
     }
   }
   return 0;
 }
 
 template <int L>
-static int sim_encode_cols_size(const uint32_t* in, uint32_t log_k, uint32_t cols, uint32_t* out) {
+static int sim_encode_cols_size(const uint32_t* in, uint32_t log_k, uint32_t cols, uint32_t* out, uint32_t chunk) {
   if constexpr (L <= 12) {
-    if (log_k == (uint32_t)L) return sim_encode_cols_plan<L>(in, cols, out);
-    return sim_encode_cols_size<L + 1>(in, log_k, cols, out);
+    if (log_k == (uint32_t)L) return sim_encode_cols_plan<L>(in, cols, out, chunk);
+    return sim_encode_cols_size<L + 1>(in, log_k, cols, out, chunk);
   } else {
     return -1;
   }
 }
 
-// The column-major encode of a whole k x cols square (k = 2^log_k, 5..12):
-// out[c * 2k + r], as the device's transpose + column kernel compute it.
-extern "C" int zoda_sim_encode_cols(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out) {
+// The column-major encode of a whole k x cols square (k = 2^log_k, 6..12,
+// cols a multiple of 64): out[c * 2k + r], as the device's transposes and
+// column kernel compute it, in chunks of `chunk` columns (a multiple of 64).
+extern "C" int zoda_sim_encode_cols(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out, uint32_t chunk) {
   uint32_t L = 0;
   while ((1u << L) < k) L++;
-  if ((1u << L) != k || L < 5 || L > 12 || cols == 0) return -1;
-  return sim_encode_cols_size<5>(in, L, cols, out);
+  if ((1u << L) != k || L < 6 || L > 12 || cols == 0 || cols % ecol::TB || chunk == 0 || chunk % ecol::TB) return -1;
+  return sim_encode_cols_size<6>(in, L, cols, out, chunk);
 }
 
 // variant < 0: the single-square plan of the side; 0..3: the batched plans.

@@ -24,7 +24,7 @@ int zo_encode(const uint32_t* input, uint32_t k, uint32_t cols, uint32_t n, uint
 int zoda_sim_ntt_fp64(uint32_t* values, uint32_t log_n, uint32_t polys, uint32_t stride, int inverse);
 int zo_ntt_babybear(uint32_t* values, uint32_t n);
 int zo_intt_babybear(uint32_t* values, uint32_t n);
-int zoda_sim_encode_cols(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out);
+int zoda_sim_encode_cols(const uint32_t* in, uint32_t k, uint32_t cols, uint32_t* out, uint32_t chunk);
 }
 
 namespace {
@@ -179,22 +179,24 @@ void check_poly_transforms() {
   }
 }
 
-// The column-major encode's column kernel (encode_cols_core.hpp), sides 2^5 ..
-// 2^12, a ragged last tile, random and extreme values.
+// The column-major encode (encode_cols_core.hpp): chunked transposes and the
+// column kernel, sides 2^6 .. 2^12, 64 columns in chunks of 64 and 192
+// columns in chunks of 128 (a partial last chunk), random and extreme values.
 void check_col_encode() {
-  for (uint32_t L = 5; L <= 12; L++) {
-    const uint32_t k = 1u << L, cols = 4096 / k + 1;
-    for (int extreme = 0; extreme < 2; extreme++) {
-      const std::vector<uint32_t> in = square(k, cols, extreme != 0);
-      std::vector<uint32_t> got((size_t)2 * k * cols), want(got.size());
-      zo_encode(in.data(), k, cols, k, want.data(), 1, 1);
-      if (zoda_sim_encode_cols(in.data(), k, cols, got.data()) != 0) {
-        printf("FAILED encode_cols k=%u\n", k);
-        g_failures++;
-        continue;
+  for (uint32_t L = 6; L <= 12; L++) {
+    const uint32_t k = 1u << L;
+    for (int extreme = 0; extreme < 2; extreme++)
+      for (uint32_t cols : {64u, 192u}) {
+        const std::vector<uint32_t> in = square(k, cols, extreme != 0);
+        std::vector<uint32_t> got((size_t)2 * k * cols), want(got.size());
+        zo_encode(in.data(), k, cols, k, want.data(), 1, 1);
+        if (zoda_sim_encode_cols(in.data(), k, cols, got.data(), cols == 64 ? 64 : 128) != 0) {
+          printf("FAILED encode_cols k=%u\n", k);
+          g_failures++;
+          continue;
+        }
+        compare("encode_cols", k, cols, 1, extreme, got, want);
       }
-      compare("encode_cols", k, cols, 1, extreme, got, want);
-    }
   }
 }
 

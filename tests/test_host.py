@@ -243,21 +243,24 @@ def test_fp64_poly_transform_emulation(log_n, inverse):
         assert np.array_equal(got, O.ntt_polys(pattern, n, n, inverse=bool(inverse)))
 
 
-@pytest.mark.parametrize("log_k", list(range(5, 13)))
+@pytest.mark.parametrize("log_k", list(range(6, 13)))
 def test_column_plan_emulation(log_k):
-    # The column-major encode's column kernel (encode_cols_core.hpp: INTT
-    # passes, the twist folded into the last of them, the middle pass, NTT
-    # passes landing in natural order), run on the host phase by phase after
-    # the transpose, against the oracle: one column, a ragged last tile, and
-    # columns of p - 1, zeros and alternating 0 / p - 1.
+    # The column-major encode (encode_cols_core.hpp) run on the host as the
+    # device schedules it: the row-major input transposed block by block into
+    # double-buffered chunks, then the column kernel (INTT passes, the twist
+    # folded into the last of them, the middle pass, NTT passes landing in
+    # natural order) over each chunk's tiles; against the oracle, for one
+    # chunk, chunks with a partial last one, and columns of p - 1, zeros and
+    # alternating 0 / p - 1.
     from helpers import sim_encode_cols
     k = 1 << log_k
     rng = np.random.default_rng(log_k)
-    for cols in (1, 4096 // k + 1):
+    for cols, chunk in ((64, None), (192, 128), (320, 64)):
         data = rng.integers(0, P, size=(k, cols), dtype=np.uint32)
-        assert np.array_equal(sim_encode_cols(data).T, O.encode(data, threads=4)), cols
-    ext = np.stack([np.full(k, P - 1), np.zeros(k), np.where(np.arange(k) % 2 == 1, P - 1, 0)], axis=1)
-    ext = ext.astype(np.uint32)
+        assert np.array_equal(sim_encode_cols(data, chunk).T, O.encode(data, threads=4)), (cols, chunk)
+    ext = np.zeros((k, 64), dtype=np.uint32)
+    ext[:, 0::3] = P - 1
+    ext[:, 2::3] = np.where(np.arange(k) % 2 == 1, P - 1, 0)[:, None]
     assert np.array_equal(sim_encode_cols(ext).T, O.encode(ext, threads=4))
 
 

@@ -1,23 +1,26 @@
 // Encode of whole squares into the column-major layout (ZODA_LAYOUT_COL,
 // out[c * 2k + r]), the reference's PinnedSquare layout (src/ntt/cuda.rs:31-67)
 // that encode_gpu fills with transpose + pad + INTT + NTT
-// (src/benchmarks/utils.rs:100-146).  Per chunk of columns, two kernels on
-// the caller's stream:
+// (src/benchmarks/utils.rs:100-146).
 //
-//   1. transpose_to_cols: the chunk of the row-major k x cols input into a
-//      column-major scratch (column c's k values contiguous), 64 x 64 tiles
-//      through LDS;
-//   2. encode_cols_kernel: per column, INTT_k, twist, NTT_k as FP64 digit
-//      passes over tiles of 4096 values in LDS (encode_cols_core.hpp), each
-//      column's 2k output values written as one contiguous run.
+// The columns go in chunks.  A chunk is first transposed from the row-major
+// k x cols input into a column-major scratch (column c's k values
+// contiguous); the column kernel then runs, per column, INTT_k, twist,
+// NTT_k as FP64 digit passes over tiles of 4096 values in LDS
+// (encode_cols_core.hpp) and writes each column's 2k output values as one
+// contiguous run.  One persistent kernel does both kinds of work: launch i
+// encodes the columns of chunk i - 1 from one scratch buffer while it
+// transposes chunk i into the other, one 64 x 64 block per column tile (the
+// counts match: a chunk of C columns has C / (4096 / k) tiles and
+// (k / 64) (C / 64) blocks), so the memory-bound transpose runs beside the
+// arithmetic instead of between its launches.
 //
-// A chunk's scratch is 128 MiB, half of MI355X's 256 MB Infinity Cache, and
-// every chunk reuses it: the transposed values go to the cache and come back
-// from it instead of making a round trip to HBM.  At side 2048 (262144
-// columns) chunks of 16384 columns take the encode from 2.81 to 2.51 ms
-// against one transpose of the whole square (alternating runs); 8192 and
-// 32768 columns came out at 2.72 and 2.69 ms.  The scratch comes from the
-// stream-ordered allocator, so encodes on different streams never share it.
+// A chunk's scratch is 64 MiB, a buffer pair half of MI355X's 256 MB Infinity
+// Cache, and the chunks reuse it: the transposed values go to the cache and
+// come back from it instead of making a round trip to HBM.
+//
+// The scratch comes from the stream-ordered allocator, so encodes on
+// different streams never share it.
 #include "common.hpp"
 #include "encode_cols_core.hpp"
 
@@ -25,89 +28,108 @@
 
 namespace zoda {
 
-constexpr int TR_SIDE = 64;    // transpose tile: 64 rows x 64 columns
-constexpr int TR_PITCH = 65;   // LDS words per tile row (conflict-free columns)
-constexpr int TR_THREADS = 256;
-constexpr uint32_t COLS_SCRATCH_BYTES = 128u << 20;  // one chunk's transposed columns
+constexpr uint32_t COLS_SCRATCH_BYTES = 64u << 20;  // one chunk's transposed columns
 
-// `pitch` u32 between input rows; the grid's tiles cover the chunk.  Tile b
-// covers rows 64 (b % tiles_r) .. +63 and columns 64 (b / tiles_r) .. +63:
-// the work-groups in flight at once read whole rows of a band of columns and
-// write one contiguous stretch of the scratch.  Each thread moves four
-// 16-byte runs in (16 lanes a tile row) and four out (16 lanes a tile
-// column).
-__global__ __launch_bounds__(TR_THREADS) void transpose_to_cols(const uint32_t* __restrict__ in, uint32_t rows,
-                                                                uint32_t pitch, uint32_t* __restrict__ t) {
-  __shared__ uint32_t sh[TR_SIDE * TR_PITCH];
-  const uint32_t tiles_r = rows / TR_SIDE;
-  const size_t r0 = (size_t)(blockIdx.x % tiles_r) * TR_SIDE;
-  const size_t c0 = (size_t)(blockIdx.x / tiles_r) * TR_SIDE;
-  const int tid = (int)threadIdx.x;
-  U4 v[4];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int id = tid + q * TR_THREADS, r = id >> 4, cc = id & 15;
-    v[q] = *reinterpret_cast<const U4*>(in + (r0 + r) * pitch + c0 + 4 * cc);
-  }
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int id = tid + q * TR_THREADS, r = id >> 4, cc = id & 15;
-    uint32_t* d = sh + r * TR_PITCH + 4 * cc;
-    d[0] = v[q].x;
-    d[1] = v[q].y;
-    d[2] = v[q].z;
-    d[3] = v[q].w;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int id = tid + q * TR_THREADS, c = id >> 4, rc = id & 15;
-    const uint32_t* s = sh + 4 * rc * TR_PITCH + c;
-    const U4 o{s[0], s[TR_PITCH], s[2 * TR_PITCH], s[3 * TR_PITCH]};
-    store16(t + (c0 + c) * rows + r0 + 4 * rc, o);
-  }
-}
+struct ColsWork {
+  const uint32_t* t;  // column-major scratch of the chunk to encode (null: none)
+  uint32_t cols;      // its columns
+  uint32_t* out;      // its output columns
+  const uint32_t* in; // row-major input of the chunk to transpose (null: none)
+  uint32_t pitch;     // u32 between input rows
+  uint32_t tcols;     // its columns (a multiple of 64)
+  uint32_t* t_next;   // its column-major scratch
+};
 
 // Persistent: each work-group copies the twiddle tables into LDS once, then
-// walks tiles of 4096 / k columns: load (each thread's sixteen values, kept
-// in registers for the store phase), passes, store.  Loading the next tile
-// into registers during the passes instead (as the drop-in transforms do)
-// measured 2.85 against 2.77 ms for the whole side-2048 encode: it takes the
-// kernel from 164 to 208 VGPRs, two waves per SIMD instead of three.
-template <int L>
-__global__ __launch_bounds__(nttf::NT) void encode_cols_kernel(const uint32_t* __restrict__ t, uint32_t cols,
-                                                               uint32_t* __restrict__ out,
-                                                               const uint32_t* __restrict__ twg) {
+// walks work items b: the transpose block b of the next chunk (through LDS
+// to the other scratch buffer), then the column tile b of this one (load --
+// each thread's sixteen values, kept in registers for the store phase --
+// passes, store).  A block's input is loaded one item ahead, before the
+// previous tile's passes, so its latency hides behind their arithmetic.  Loading the next tile into registers during the passes instead
+// (as the drop-in transforms do) takes the kernel from 164 to 208 VGPRs, two
+// waves per SIMD instead of three: 2.85 against 2.77 ms for the whole
+// side-2048 encode, measured with separate transpose launches.
+// MODE: COLS_BOTH (encode the tiles of one chunk and transpose the blocks of
+// the next; the next chunk never has more blocks than this one has tiles),
+// COLS_TRANSPOSE (the first launch) or COLS_ENCODE (the last).
+enum { COLS_BOTH = 0, COLS_TRANSPOSE = 1, COLS_ENCODE = 2 };
+
+template <int L, int MODE>
+__global__ __launch_bounds__(nttf::NT) void encode_cols_kernel(ColsWork w, const uint32_t* __restrict__ twg) {
   using PL = ecol::Plan<L>;
+  static_assert(ecol::TB % PL::COLS == 0, "chunks of whole transpose blocks are whole tiles");
+  constexpr bool TR = MODE != COLS_ENCODE, COL = MODE != COLS_TRANSPOSE;
   extern __shared__ int32_t lds[];
   uint32_t* tw = reinterpret_cast<uint32_t*>(lds + nttf::LDS_DATA);
+  uint32_t* sh = reinterpret_cast<uint32_t*>(lds);  // a transpose block, between tiles
   const int tid = (int)threadIdx.x;
-  for (int i = tid; i < PL::TW_WORDS; i += nttf::NT) tw[i] = twg[i];
+  if constexpr (COL)
+    for (int i = tid; i < PL::TW_WORDS; i += nttf::NT) tw[i] = twg[i];
   const slice::MulK k = slice::make_mulk();
-  const uint32_t ntiles = (cols + PL::COLS - 1) / PL::COLS;
-  auto load2 = [&](size_t c, uint32_t j) -> ecol::U2 {
-    return *reinterpret_cast<const ecol::U2*>(t + c * PL::S + j);
+  const uint32_t ntiles = COL ? w.cols / PL::COLS : 0u;
+  const uint32_t nblocks = TR ? (uint32_t)(PL::S / ecol::TB) * (w.tcols / ecol::TB) : 0u;
+  const uint32_t nitems = COL ? ntiles : nblocks;
+  // Raw-buffer accesses: a per-lane 32-bit byte offset and a uniform one
+  // (every byte offset stays below 2^32: a chunk's scratch is 64 MiB, its
+  // output 128 MiB, a transpose block's input rows at most 63 pitches apart
+  // with pitch < 2^24), so no 64-bit address stays live across the loop.
+  // Every memory access in the loop is unconditional: an item past the end
+  // of its kind goes through a buffer of zero records (loads return 0,
+  // stores are dropped).  A branch around one would leave the compiler's
+  // count of outstanding accesses conservative after it, and the next wait
+  // would then cover loads issued to be consumed much later.
+  const GBuf tin(w.t), tout(w.out);
+  const uint32_t lane_in = 8u * tid, lane_out = 16u * tid;
+  const uint32_t tb_lane_in = 4u * (uint32_t)ecol::tb_in_offset(tid, 0, w.pitch);
+  const uint32_t tb_lane_out = 4u * (uint32_t)ecol::tb_out_offset(tid, 0, PL::S);
+  // transpose block b's input, loaded one item ahead
+  U4 tv[ecol::TB_RUNS];
+  auto fetch_block = [&](uint32_t b) {
+    const ecol::TBlock tb = ecol::tblock(b < nblocks ? b : 0u, PL::S);
+    const GBuf src(w.in + (size_t)tb.r0 * w.pitch + tb.c0, b < nblocks);
+#pragma unroll
+    for (int q = 0; q < ecol::TB_RUNS; q++) tv[q] = src.load4(tb_lane_in, 4u * q * (nttf::NT / 16) * w.pitch);
   };
-  // plain u32 arrays: an array of U2 is left in scratch memory by the
-  // compiler, and every store then waits on all outstanding loads
-  uint32_t cx[ecol::PAIRS], cy[ecol::PAIRS];
-  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const size_t col0 = (size_t)tile * PL::COLS;
-    const uint32_t live = min((uint32_t)PL::COLS, cols - (uint32_t)col0);
+  if constexpr (TR) fetch_block(blockIdx.x);
+  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    // Each pair stays one 64-bit register pair from its load to the store
+    // phase: split earlier, the compiler moves the second value into its
+    // place in the 16-byte store's operand right after the load, which makes
+    // the wave wait for the load there.  (An array of U2 structs is worse: the
+    // compiler leaves it in scratch memory, and every store then waits on all
+    // outstanding loads.)
+    uint64_t pr[ecol::PAIRS];
+    const uint32_t tbase = it * (uint32_t)(nttf::TILE * 4);  // the tile's first scratch byte
+    if constexpr (COL) {
 #pragma unroll
-    for (int q = 0; q < ecol::PAIRS; q++) {
-      const ecol::U2 x = ecol::tile_pair<PL>(tid, q, live, col0, load2);
-      cx[q] = x.x;
-      cy[q] = x.y;
+      for (int q = 0; q < ecol::PAIRS; q++) pr[q] = tin.load2(lane_in, tbase + 8u * q * nttf::NT);
     }
-    __syncthreads();  // the tables, or the previous tile's last reads of LDS
+    if constexpr (TR) {
+      const ecol::TBlock tb = ecol::tblock(it < nblocks ? it : 0u, PL::S);
+      const GBuf dst(w.t_next + (size_t)tb.c0 * PL::S + tb.r0, it < nblocks);
+      __syncthreads();  // the previous tile's last reads of LDS
 #pragma unroll
-    for (int q = 0; q < ecol::PAIRS; q++) ecol::put_pair(tid, q, lds, ecol::U2{cx[q], cy[q]});
-    __syncthreads();
-    ecol::all_passes<PL>([&](auto pass) { pass(tid, lds, tw, k); }, [] { __syncthreads(); });
-    ecol::store_pairs<PL>(
-        tid, live, col0, lds, [&](int q) { return ecol::U2{cx[q], cy[q]}; },
-        [&](size_t c, uint32_t r, const U4& v) { store16<true>(out + c * (2 * PL::S) + r, v); });
+      for (int q = 0; q < ecol::TB_RUNS; q++) ecol::tb_put(tid, q, sh, tv[q]);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < ecol::TB_RUNS; q++)
+        dst.store4_cached(ecol::tb_get(tid, q, sh), tb_lane_out, 4u * q * (nttf::NT / 16) * PL::S);
+      fetch_block(it + gridDim.x);  // arrives during this tile's passes
+    }
+    if constexpr (COL) {
+      __syncthreads();  // the tables, the block's or the previous tile's last reads of LDS
+#pragma unroll
+      for (int q = 0; q < ecol::PAIRS; q++) ecol::put_pair(tid, q, lds, ecol::U2{(uint32_t)pr[q], (uint32_t)(pr[q] >> 32)});
+      __syncthreads();
+      ecol::all_passes<PL>([&](auto pass) { pass(tid, lds, tw, k); }, [] { __syncthreads(); });
+      ecol::store_pairs<PL>(
+          tid, (uint32_t)PL::COLS, (size_t)it * PL::COLS, lds,
+          [&](int q) {
+            asm volatile("" : "+v"(pr[q]));
+            return ecol::U2{(uint32_t)pr[q], (uint32_t)(pr[q] >> 32)};
+          },
+          [&](int q, size_t, uint32_t, const U4& v) { tout.store4(v, lane_out, 2u * tbase + 16u * q * nttf::NT); });
+    }
   }
 }
 
@@ -134,28 +156,42 @@ static int launch_cols_plan(DeviceState* st, const uint32_t* d_in, uint32_t cols
     }
     twg = it->second;
   }
-  const void* fn = (const void*)encode_cols_kernel<L>;
+  const void* fns[3] = {(const void*)encode_cols_kernel<L, COLS_BOTH>, (const void*)encode_cols_kernel<L, COLS_TRANSPOSE>,
+                        (const void*)encode_cols_kernel<L, COLS_ENCODE>};
   const size_t lds_bytes = (size_t)PL::LDS_WORDS * sizeof(int32_t);
   uint32_t grid = 0;
-  ZODA_TRY(kernel_setup(st, fn, nttf::NT, lds_bytes, true, &grid));
+  for (const void* f : fns) ZODA_TRY(kernel_setup(st, f, nttf::NT, lds_bytes, true, &grid));
 
   const uint32_t k = PL::S;
   const uint32_t chunk = std::min<uint32_t>(cols, COLS_SCRATCH_BYTES / ((uint32_t)sizeof(uint32_t) * k));
+  const uint32_t nchunks = (cols + chunk - 1) / chunk;
   uint32_t* scratch = nullptr;
-  ZODA_HIP(hipMallocAsync((void**)&scratch, (size_t)k * chunk * sizeof(uint32_t), stream));
+  const size_t buf = (size_t)k * chunk;  // u32 per buffer
+  ZODA_HIP(hipMallocAsync((void**)&scratch, 2 * buf * sizeof(uint32_t), stream));
   hipError_t e = hipSuccess;
-  for (uint32_t c0 = 0; c0 < cols && e == hipSuccess; c0 += chunk) {
-    const uint32_t n = std::min(chunk, cols - c0);
-    const unsigned tr_blocks = (k / TR_SIDE) * (n / TR_SIDE);
-    hipLaunchKernelGGL(transpose_to_cols, dim3(tr_blocks), dim3(TR_THREADS), 0, stream, d_in + c0, k, cols, scratch);
-    e = hipGetLastError();
-    if (e == hipSuccess) {
-      const uint32_t* cs = scratch;
-      uint32_t* o = d_out + (size_t)c0 * 2 * k;
-      const uint32_t g = std::min(grid, (n + PL::COLS - 1) / PL::COLS);
-      void* args[] = {(void*)&cs, (void*)&n, (void*)&o, (void*)&twg};
-      e = hipLaunchKernel(fn, dim3(g), dim3(nttf::NT), args, lds_bytes, stream);
+  // launch i: encode chunk i - 1 (buffer (i - 1) % 2), transpose chunk i (buffer i % 2)
+  for (uint32_t i = 0; i <= nchunks && e == hipSuccess; i++) {
+    ColsWork w{};
+    uint32_t items = 0;
+    if (i > 0) {
+      const uint32_t c0 = (i - 1) * chunk;
+      w.t = scratch + ((i - 1) % 2) * buf;
+      w.cols = std::min(chunk, cols - c0);
+      w.out = d_out + (size_t)c0 * 2 * k;
+      items = w.cols / PL::COLS;
     }
+    if (i < nchunks) {
+      const uint32_t c0 = i * chunk;
+      w.in = d_in + c0;
+      w.pitch = cols;
+      w.tcols = std::min(chunk, cols - c0);
+      w.t_next = scratch + (i % 2) * buf;
+      if (i == 0) items = (k / ecol::TB) * (w.tcols / ecol::TB);
+    }
+    const void* fn = fns[i == 0 ? COLS_TRANSPOSE : i == nchunks ? COLS_ENCODE : COLS_BOTH];
+    const uint32_t g = std::min(grid, items);
+    void* args[] = {(void*)&w, (void*)&twg};
+    e = hipLaunchKernel(fn, dim3(g), dim3(nttf::NT), args, lds_bytes, stream);
   }
   const hipError_t ef = hipFreeAsync(scratch, stream);
   if (e != hipSuccess) return fail((int)e, "column encode launch failed: %s", hipGetErrorString(e));
@@ -175,7 +211,7 @@ static int launch_cols_size(DeviceState* st, const uint32_t* d_in, uint32_t log_
 }
 
 bool cols_fits(const uint32_t* d_in, uint32_t k, uint32_t cols, const uint32_t* d_out, hipStream_t stream) {
-  if (k < 2048 || k > 4096 || (k & (k - 1)) || cols == 0 || cols % TR_SIDE) return false;
+  if (k < 2048 || k > 4096 || (k & (k - 1)) || cols == 0 || cols % ecol::TB || cols >= (1u << 24)) return false;
   if (((uintptr_t)d_in & 15) || ((uintptr_t)d_out & 15)) return false;
   // the stream-ordered scratch stays out of graph capture
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;

@@ -267,6 +267,55 @@ ZHD void all_passes(const Each& each, const Sync& sync) {
   sync();
 }
 
+// The transpose of the row-major input into the column-major scratch, in
+// blocks of 64 rows x 64 columns through LDS (64 x 65 words, so a block
+// column is read from 32 different banks).  Block b of a chunk covers rows
+// 64 (b % (k / 64)) .. +63 and columns 64 (b / (k / 64)) .. +63 (the blocks in
+// flight at once read whole rows of a band of columns and write one
+// contiguous stretch of the scratch).  Thread tid's run q (q < 4): in, row
+// id / 16 of the block, columns 4 (id % 16) .. +3; out, column id / 16, rows
+// 4 (id % 16) .. +3 (id = tid + q NT).
+constexpr int TB = 64, TB_PITCH = 65, TB_RUNS = TB * TB / 4 / NT;
+static_assert(TB * TB_PITCH <= nttf::LDS_DATA, "a transpose block fits the tile's LDS");
+
+struct TBlock {
+  uint32_t r0, c0;  // first row and column of the block within the chunk
+};
+
+ZHD TBlock tblock(uint32_t b, uint32_t k) {
+  const uint32_t tiles_r = k / TB;
+  return TBlock{(b % tiles_r) * TB, (b / tiles_r) * TB};
+}
+
+// run q of thread tid: offset of its 16 input bytes from the block's first
+// input value (pitch u32 a row), and its LDS words
+ZHD size_t tb_in_offset(int tid, int q, uint32_t pitch) {
+  const int id = tid + q * NT;
+  return (size_t)(id >> 4) * pitch + 4 * (id & 15);
+}
+
+ZHD void tb_put(int tid, int q, uint32_t* __restrict__ sh, const U4& v) {
+  const int id = tid + q * NT;
+  uint32_t* d = sh + (id >> 4) * TB_PITCH + 4 * (id & 15);
+  d[0] = v.x;
+  d[1] = v.y;
+  d[2] = v.z;
+  d[3] = v.w;
+}
+
+// run q of thread tid on the way out: the four values and their offset from
+// the block's first scratch value (k u32 a column)
+ZHD U4 tb_get(int tid, int q, const uint32_t* __restrict__ sh) {
+  const int id = tid + q * NT;
+  const uint32_t* s = sh + 4 * (id & 15) * TB_PITCH + (id >> 4);
+  return U4{s[0], s[TB_PITCH], s[2 * TB_PITCH], s[3 * TB_PITCH]};
+}
+
+ZHD size_t tb_out_offset(int tid, int q, uint32_t k) {
+  const int id = tid + q * NT;
+  return (size_t)(id >> 4) * k + 4 * (id & 15);
+}
+
 // A thread's share of a tile: the value pairs (rows j, j + 1 of one column)
 // at tile index 2 (tid + q NT), q < PAIRS.  They are loaded as 8-byte runs of
 // the column-major input, put into LDS, and kept in registers for the store
@@ -302,7 +351,10 @@ ZHD void put_pair(int tid, int q, int32_t* __restrict__ lds, const U2& x) {
 }
 
 // Store phase: pair q (held(q) gives its input values) becomes output rows
-// 2j .. 2j + 3 of its column, one 16-byte store (x_j, y_j, x_(j+1), y_(j+1)).
+// 2j .. 2j + 3 of its column, one 16-byte store (x_j, y_j, x_(j+1), y_(j+1)),
+// handed over as store4(q, column, row, values).  Within a whole tile, pair
+// q's input is value 2 (tid + q NT) of the tile's column-major values and its
+// output starts at value 4 (tid + q NT) of the tile's output.
 template <class PL, class Held, class Store4>
 ZHD void store_pairs(int tid, uint32_t live_cols, size_t col0, const int32_t* __restrict__ lds, const Held& held,
                      const Store4& store4) {
@@ -318,7 +370,7 @@ ZHD void store_pairs(int tid, uint32_t live_cols, size_t col0, const int32_t* __
       o.y = (uint32_t)lds[phys(v)];
       o.z = x.y;
       o.w = (uint32_t)lds[phys(v + 1)];
-      store4(col0 + (size_t)cl, (uint32_t)(2 * j), o);
+      store4(q, col0 + (size_t)cl, (uint32_t)(2 * j), o);
     }
   }
 }

@@ -260,6 +260,16 @@ struct GBuf {
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, v), r, voff, soff, 2);
     asm volatile("s_waitcnt expcnt(0)\n\ts_nop 1" ::: "memory");
   }
+  // the same with the default cache policy, for values read again soon
+  __device__ void store4_cached(const U4& v, uint32_t voff, uint32_t soff) const {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, v), r, voff, soff, 0);
+    asm volatile("s_waitcnt expcnt(0)\n\ts_nop 1" ::: "memory");
+  }
+  // 8-byte load (two adjacent u32, the first in the low half)
+  using V2 = __attribute__((ext_vector_type(2))) unsigned int;
+  __device__ uint64_t load2(uint32_t voff, uint32_t soff) const {
+    return __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  }
 #else
   char* b;
   ZHD explicit GBuf(const uint32_t* p) : b(reinterpret_cast<char*>(const_cast<uint32_t*>(p))) {}
@@ -278,6 +288,10 @@ struct GBuf {
   }
   ZHD void store4(const U4& v, uint32_t voff, uint32_t soff) const {
     if (b) *reinterpret_cast<U4*>(b + ((size_t)voff + soff)) = v;
+  }
+  ZHD void store4_cached(const U4& v, uint32_t voff, uint32_t soff) const { store4(v, voff, soff); }
+  ZHD uint64_t load2(uint32_t voff, uint32_t soff) const {
+    return b ? *reinterpret_cast<const uint64_t*>(b + ((size_t)voff + soff)) : 0u;
   }
 #endif
 };
