@@ -401,28 +401,29 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
   for (uint32_t r = 0; r < n; r++)
     if (!known[r]) rows.push_back(r);
 
-  // One device block: x_m | hpow_m | hinv_m | ep_m | rows | zk_m | zinv | known.
+  // One device block: rows | x_m | hpow_m | hinv_m | ep_m | zk_m | zinv | known
+  // (rows first: dec_columns reads it 16 bytes at a time).
   hipStream_t s = (hipStream_t)stream;
-  const size_t up_words = (size_t)n + n + k + ne + n;
+  const size_t up_words = (size_t)n + n + n + k + ne;
   const size_t words = up_words + n + n;
   uint32_t* d_blk = nullptr;
   uint32_t* T = nullptr;
   ZODA_HIP(hipMalloc((void**)&d_blk, words * 4 + n));
-  uint32_t* d_x = d_blk;
+  uint32_t* d_rows = d_blk;
+  uint32_t* d_x = d_rows + n;
   uint32_t* d_hpow = d_x + n;
   uint32_t* d_hinv = d_hpow + n;
   uint32_t* d_ep = d_hinv + k;
-  uint32_t* d_rows = d_ep + ne;
-  uint32_t* d_zk = d_rows + n;
+  uint32_t* d_zk = d_ep + ne;
   uint32_t* d_zinv = d_zk + n;
   uint8_t* d_known = reinterpret_cast<uint8_t*>(d_zinv + n);
   std::vector<uint32_t> host;
   host.reserve(up_words);
+  host.insert(host.end(), rows.begin(), rows.end());
   host.insert(host.end(), x_m.begin(), x_m.end());
   host.insert(host.end(), hpow_m.begin(), hpow_m.end());
   host.insert(host.end(), hinv_m.begin(), hinv_m.end());
   host.insert(host.end(), ep_m.begin(), ep_m.end());
-  host.insert(host.end(), rows.begin(), rows.end());
   int rc = 0;
   hip_note(rc, hipMemcpy(d_blk, host.data(), up_words * 4, hipMemcpyHostToDevice), "decode vectors upload");
   if (!rc) hip_note(rc, hipMemcpy(d_known, known.data(), n, hipMemcpyHostToDevice), "decode vectors upload");

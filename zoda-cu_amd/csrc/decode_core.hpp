@@ -55,7 +55,10 @@ struct DecodePlan {
   static constexpr int S = 1 << L;  // n
   static constexpr int LO1 = RD + R2, LO2 = RD;
   // LDS word of (row, column): one pad word every 2^RD rows, so the middle
-  // pass's lanes (2^RD rows apart) fall into different banks.
+  // pass's lanes (2^RD rows apart) fall into different banks.  (Pads chosen
+  // to clear the remaining two-way conflicts of the NTT passes as well cost
+  // more in address arithmetic than they saved: 8.02-8.10 against 7.70 ms
+  // per 2048^2 decode.)
   static constexpr int TILE_VALUES = W * S;
   static constexpr int TILE_WORDS = W * S + (S >> RD);
   // The top digit's twiddles w^(+-low k), low < 2^LO1: one table of n
@@ -155,7 +158,7 @@ ZHD void dec_mid(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict_
     for (int q = 0; q < H; q++) {
       const uint32_t j = row_of<0, R>(u, 2u * (uint32_t)q);
       const uint32_t k = (j >> LOA) & ((1u << RA) - 1u);
-      y[q] = mulmod(y[q], tw_get<true>(above, (k << LOA) | (j & ((1u << LOA) - 1u))));
+      y[q] = mulmod(y[q], tw_get<true>(above, ((j & ((1u << LOA) - 1u)) << RA) | k));
       lds[lds_at<PL>(j, c)] = (int32_t)y[q];
     }
   }
@@ -217,34 +220,61 @@ ZHD void dec_ntt1(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
   }
 }
 
-// Tile `tile` of the scratch square into LDS.  The scratch holds, per column,
-// only the k = n/2 known rows, by rank: entry c * k + i is y at row rows[i]
-// (the known rows in increasing order); rows[k + i] are the erased rows,
-// which get 0.  Columns past `cols` (a ragged last tile) are zero.
+// 16-byte global accesses (the store with the wait states of GBuf::store4).
+ZHD U4 dec_ld16(const uint32_t* p) { return *reinterpret_cast<const U4*>(p); }
+ZHD void dec_st16(uint32_t* p, const U4& v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  store16(p, v);
+#else
+  *reinterpret_cast<U4*>(p) = v;
+#endif
+}
+
+// Tile `tile` (W columns) into LDS: the k packed known rows of each column
+// (scratch entry c * k + q is rank q, at row rows[q]; the known rows in
+// increasing order) and zeros at the erased rows (rows[k + i]), four
+// ranks at a time: a 16-byte load of the values and one of their row
+// numbers (`rows` 16-byte aligned, the same for every tile), scattered into
+// LDS.  Columns past `cols` get zeros.
 template <class PL>
 ZHD void dec_load(int tid, uint32_t tile, uint32_t cols, const uint32_t* __restrict__ T,
                   const uint32_t* __restrict__ rows, int32_t* __restrict__ lds) {
-  constexpr uint32_t K = PL::S / 2;
+  constexpr uint32_t K = PL::S / 2, KQ = K / 4;
+  static_assert(K % 4 == 0, "ranks come four at a time");
   const uint32_t* src = T + (size_t)tile * PL::W * K;
-  const uint32_t valid = (cols - tile * PL::W < (uint32_t)PL::W ? cols - tile * PL::W : (uint32_t)PL::W) * K;
-  for (uint32_t i = (uint32_t)tid; i < (uint32_t)PL::TILE_VALUES; i += PL::NT) {
-    const uint32_t c = i / (uint32_t)PL::S, q = i % (uint32_t)PL::S;  // q < K: known, else erased
-    const uint32_t r = rows[q];
-    const uint32_t idx = c * K + q;
-    lds[lds_at<PL>(r, c)] = (q < K && idx < valid) ? (int32_t)src[idx] : 0;
+  const uint32_t wv = cols - tile * PL::W < (uint32_t)PL::W ? cols - tile * PL::W : (uint32_t)PL::W;
+  for (uint32_t i = (uint32_t)tid; i < (uint32_t)PL::W * KQ; i += PL::NT) {
+    const uint32_t c = i / KQ, q = (i % KQ) * 4;
+    const U4 r = dec_ld16(rows + q);
+    U4 v{0, 0, 0, 0};
+    if (c < wv) v = dec_ld16(src + (size_t)c * K + q);
+    lds[lds_at<PL>(r.x, c)] = (int32_t)v.x;
+    lds[lds_at<PL>(r.y, c)] = (int32_t)v.y;
+    lds[lds_at<PL>(r.z, c)] = (int32_t)v.z;
+    lds[lds_at<PL>(r.w, c)] = (int32_t)v.w;
+  }
+  for (uint32_t i = (uint32_t)tid; i < (uint32_t)PL::W * KQ; i += PL::NT) {
+    const uint32_t c = i / KQ, q = K + (i % KQ) * 4;
+    const U4 r = dec_ld16(rows + q);
+    lds[lds_at<PL>(r.x, c)] = 0;
+    lds[lds_at<PL>(r.y, c)] = 0;
+    lds[lds_at<PL>(r.z, c)] = 0;
+    lds[lds_at<PL>(r.w, c)] = 0;
   }
 }
 
 // The data rows (the even rows) of tile `tile` from LDS into the k entries of
-// each scratch column.
+// each scratch column, four rows to a 16-byte store.
 template <class PL>
 ZHD void dec_store(int tid, uint32_t tile, uint32_t cols, uint32_t* __restrict__ T, const int32_t* __restrict__ lds) {
-  constexpr uint32_t K = PL::S / 2;
+  constexpr uint32_t K = PL::S / 2, KQ = K / 4;
   uint32_t* dst = T + (size_t)tile * PL::W * K;
   const uint32_t wv = cols - tile * PL::W < (uint32_t)PL::W ? cols - tile * PL::W : (uint32_t)PL::W;
-  for (uint32_t i = (uint32_t)tid; i < wv * K; i += PL::NT) {
-    const uint32_t c = i / K, j = i % K;
-    dst[i] = (uint32_t)lds[lds_at<PL>(2 * j, c)];
+  for (uint32_t i = (uint32_t)tid; i < wv * KQ; i += PL::NT) {
+    const uint32_t c = i / KQ, j = (i % KQ) * 4;
+    const U4 o{(uint32_t)lds[lds_at<PL>(2 * j, c)], (uint32_t)lds[lds_at<PL>(2 * j + 2, c)],
+               (uint32_t)lds[lds_at<PL>(2 * j + 4, c)], (uint32_t)lds[lds_at<PL>(2 * j + 6, c)]};
+    dec_st16(dst + (size_t)c * K + j, o);
   }
 }
 
@@ -279,6 +309,24 @@ struct DecodeTableImage {
   std::vector<uint32_t> packed;
   uint32_t o_inv1 = 0, o_fwd1 = 0, o_fwd2 = 0, o_inv2 = 0;
 };
+
+// Forward twiddles of the digit of r bits at bit lo in [low << r | k]
+// order: w^(low k), w the root of order 2^(lo+r).  The middle pass reads
+// them with k running over the lanes (k << lo | low would put its 16 lanes'
+// words 16 apart: eight-way bank conflicts).
+inline void push_forward_low_major(std::vector<Tw>& v, int lo, int r) {
+  const uint32_t w = root_of_unity((uint32_t)(lo + r));
+  const size_t base = v.size();
+  v.resize(base + ((size_t)1 << (lo + r)));
+  for (uint32_t k = 0; k < (1u << r); k++) {
+    const uint32_t step = pow_mod_host(w, k);
+    uint32_t cur = 1;
+    for (uint32_t low = 0; low < (1u << lo); low++) {
+      v[base + (((size_t)low << r) | k)] = make_tw(cur);
+      cur = mul_mod_host(cur, step);
+    }
+  }
+}
 
 // Inverse twiddles of the digit of r bits at bit lo in [k << lo | low]
 // order: w^-(low k), w the root of order 2^(lo+r).
@@ -323,7 +371,7 @@ DecodeTableImage build_decode_tables() {
   im.o_fwd1 = (uint32_t)t.size();
   push_top_digit(t, lo1, r1, false, PL::FT, PL::FA);
   im.o_fwd2 = (uint32_t)t.size();
-  push_digit_table(t, lo2, r2, false);
+  push_forward_low_major(t, lo2, r2);
   im.o_inv2 = (uint32_t)t.size();
   push_inverse_digit(t, lo2, r2);
   for (const Tw& x : t) im.packed.push_back((uint32_t)x.w);
