@@ -46,6 +46,26 @@ def test_decode_from_any_k_rows(zh, torch, k, cols):
         assert torch.equal(got, t_in), (k, cols, name)
 
 
+@pytest.mark.parametrize("k,cols", [(128, 320), (256, 192), (512, 64), (1024, 192), (2048, 128), (4096, 4480),
+                                    (8192, 2240)])
+def test_decode_chunked_path(zh, torch, k, cols):
+    # whole 64-column blocks run the chunked FP64 path (gather, column kernel
+    # and scatter of successive chunks in one launch); 4096 x 4480 and
+    # 8192 x 2240 span two chunks, the second partial
+    rng = np.random.default_rng(k * 11 + cols)
+    t_in = torch.empty((k, cols), dtype=torch.int32, device="cuda")
+    zh.fill_uniform(t_in, seed=k * 3 + cols)
+    enc = zh.encode_tensor(t_in)
+    for name, known in patterns(k, rng):
+        print(f"[case] chunked decode k={k} cols={cols} rows={name}", flush=True)
+        damaged = enc.clone()
+        erased = np.setdiff1d(np.arange(2 * k), known)
+        damaged[torch.from_numpy(erased.astype(np.int64)).cuda()] = -1
+        got = zh.decode_tensor(damaged, rng.permutation(known))
+        torch.cuda.synchronize()
+        assert torch.equal(got, t_in), (k, cols, name)
+
+
 def test_decode_2048_square(zh, torch):
     side, cols = 2048, 128 * 2048
     print(f"[case] decode {side}x{cols}", flush=True)

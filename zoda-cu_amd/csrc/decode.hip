@@ -271,6 +271,118 @@ __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_columns(uint32_t* __rest
   }
 }
 
+// ---- chunked form of the FP64 path (whole 64-column blocks) -------------------
+//
+// The columns go in chunks through three scratch buffers; launch i gathers
+// chunk i, runs the column kernel on chunk i - 1 and scatters chunk i - 2, so
+// the memory-bound transposes run beside the VALU-bound column work instead
+// of in launches of their own.  A gather block is 64 known ranks x 64
+// columns, a scatter block 64 data rows x 64 columns, each through LDS (the
+// tile's words, 64 x 65); item b of a launch is gather block b, column tile b
+// and scatter block b, whichever exist.
+
+constexpr int DB = 64, DB_PITCH = 65;  // transpose block of the chunked path
+
+struct DecChunks {
+  const uint32_t* enc;  // extended square, row-major (cols a row)
+  uint32_t* data;       // data square, row-major
+  uint32_t cols, k;
+  const uint32_t* rows; // known rows by rank, then the erased rows
+  const uint32_t* zk;   // Z(w^r), Montgomery form
+  const uint8_t* known; // known[r]
+  uint32_t* tg;         // gather: scratch of chunk g (null: none), its columns and first column
+  uint32_t gcols, gc0;
+  uint32_t* tc;         // column kernel: scratch of chunk c (null: none), its columns
+  uint32_t ccols;
+  const uint32_t* ts;   // scatter: scratch of chunk s (null: none), its columns and first column
+  uint32_t scols, sc0;
+};
+
+template <class PL>
+__global__ __launch_bounds__(PL::NT, PL::MINW) void dec_chunks(DecChunks a, const uint32_t* __restrict__ srow,
+                                                               DecodeTabs tabs) {
+  static_assert(DB * DB_PITCH <= PL::TILE_WORDS, "a transpose block fits the tile's LDS");
+  constexpr uint32_t K = PL::S / 2, KB = K / DB;  // rank (or data-row) blocks per column band
+  extern __shared__ int32_t lds[];
+  uint32_t* sh = reinterpret_cast<uint32_t*>(lds);
+  __shared__ uint32_t rid[DB], zr[DB];
+  const int tid = (int)threadIdx.x;
+  dec_load_tables<PL>(tid, lds, tabs);
+  DecodeTabs ct = tabs;  // the roots from constant memory
+  ct.roots = kDecodeConst.roots;
+  const uint32_t ng = a.tg ? KB * (a.gcols / DB) : 0u;
+  const uint32_t ntile = a.tc ? a.ccols / PL::W : 0u;
+  const uint32_t ns = a.ts ? KB * (a.scols / DB) : 0u;
+  const uint32_t nitems = max(ng, max(ntile, ns));
+  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    if (it < ng) {  // gather: ranks i0.., columns cb.. of chunk g
+      const uint32_t i0 = (it % KB) * DB, cb = (it / KB) * DB;
+      __syncthreads();  // the previous item's last reads of LDS
+      if (tid < DB) {
+        const uint32_t r = a.rows[i0 + tid];
+        rid[tid] = r;
+        zr[tid] = a.zk[r];
+      }
+      __syncthreads();
+      for (int idx = tid; idx < DB * DB / 4; idx += PL::NT) {
+        const uint32_t q = idx / 16, c4 = 4 * (idx % 16), r = rid[q], z = zr[q];
+        const size_t col = (size_t)a.gc0 + cb + c4;
+        const U4 e = *reinterpret_cast<const U4*>(a.enc + (size_t)r * a.cols + col);
+        if (!(r & 1u)) store16(a.data + (size_t)(r >> 1) * a.cols + col, e);  // a known data row
+        uint32_t* d = sh + q * DB_PITCH + c4;
+        d[0] = mont_mul(e.x, z);
+        d[1] = mont_mul(e.y, z);
+        d[2] = mont_mul(e.z, z);
+        d[3] = mont_mul(e.w, z);
+      }
+      __syncthreads();
+      for (int idx = tid; idx < DB * DB / 4; idx += PL::NT) {
+        const uint32_t c = idx / 16, q4 = 4 * (idx % 16);
+        const uint32_t* s = sh + q4 * DB_PITCH + c;
+        store16(a.tg + (size_t)(cb + c) * K + i0 + q4, U4{s[0], s[DB_PITCH], s[2 * DB_PITCH], s[3 * DB_PITCH]});
+      }
+    }
+    if (it < ntile) {  // the column kernel on tile it of chunk c
+      int tid2 = tid;
+      asm("" : "+v"(tid2) : "s"(it));  // keep the per-thread address arithmetic in the loop
+      __syncthreads();  // the tables, or the previous reads of LDS
+      dec_load<PL>(tid2, it, a.ccols, a.tc, a.rows, lds);
+      __syncthreads();
+      dec_step<PL, 0>(tid2, lds, ct, srow);
+      __syncthreads();
+      dec_step<PL, 1>(tid2, lds, ct, srow);
+      __syncthreads();
+      dec_step<PL, 2>(tid2, lds, ct, srow);
+      __syncthreads();
+      dec_step<PL, 3>(tid2, lds, ct, srow);
+      __syncthreads();
+      dec_step<PL, 4>(tid2, lds, ct, srow);
+      __syncthreads();
+      dec_store<PL>(tid2, it, a.ccols, a.tc, lds);
+    }
+    if (it < ns) {  // scatter: data rows j0.., columns cb.. of chunk s (the erased ones)
+      const uint32_t j0 = (it % KB) * DB, cb = (it / KB) * DB;
+      __syncthreads();
+      for (int idx = tid; idx < DB * DB / 4; idx += PL::NT) {
+        const uint32_t c = idx / 16, j4 = 4 * (idx % 16);
+        const U4 v = *reinterpret_cast<const U4*>(a.ts + (size_t)(cb + c) * K + j0 + j4);
+        uint32_t* d = sh + j4 * DB_PITCH + c;
+        d[0] = v.x;
+        d[DB_PITCH] = v.y;
+        d[2 * DB_PITCH] = v.z;
+        d[3 * DB_PITCH] = v.w;
+      }
+      __syncthreads();
+      for (int idx = tid; idx < DB * DB / 4; idx += PL::NT) {
+        const uint32_t j = idx / 16, c4 = 4 * (idx % 16);
+        if (a.known[2 * (j0 + j)]) continue;  // written by the gather
+        const uint32_t* s = sh + j * DB_PITCH + c4;
+        store16(a.data + (size_t)(j0 + j) * a.cols + a.sc0 + cb + c4, U4{s[0], s[1], s[2], s[3]});
+      }
+    }
+  }
+}
+
 // Tables of the FP64 decoder plan PL, built and uploaded once per device.
 template <class PL>
 int get_decode_tables(DeviceState* st, const DecodeTabs** out) {
@@ -314,6 +426,69 @@ int decode_grid(DeviceState* st, uint32_t* grid) {
   }
   *grid = it->second;
   return 0;
+}
+
+// Once per device: the LDS limit of dec_chunks<PL> and its persistent grid.
+template <class PL>
+int decode_chunks_grid(DeviceState* st, uint32_t* grid) {
+  const void* fn = (const void*)dec_chunks<PL>;
+  std::lock_guard<std::recursive_mutex> lock(st->mu);
+  auto it = st->kernel_setup.find(fn);
+  if (it == st->kernel_setup.end()) {
+    constexpr size_t lds = PL::LDS_WORDS * sizeof(int32_t);
+    if (lds > 64 * 1024) ZODA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0, cus = 0;
+    ZODA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, PL::NT, lds));
+    ZODA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, st->device));
+    it = st->kernel_setup.emplace(fn, (uint32_t)std::max(1, per_cu * cus)).first;
+  }
+  *grid = it->second;
+  return 0;
+}
+
+// The chunked FP64 decode (dec_chunks): chunks of DEC_CHUNK_BYTES of scratch
+// each, three buffers, nchunks + 2 launches.
+constexpr size_t DEC_CHUNK_BYTES = 64u << 20;
+
+template <class PL>
+int decode_chunked(DeviceState* st, DecChunks base, const uint32_t* srow, const DecodeTabs& tabs, hipStream_t s) {
+  constexpr uint32_t K = PL::S / 2;
+  uint32_t grid = 0;
+  ZODA_TRY(decode_chunks_grid<PL>(st, &grid));
+  const uint32_t cols = base.cols;
+  uint32_t chunk = (uint32_t)std::min<size_t>(cols, DEC_CHUNK_BYTES / (sizeof(uint32_t) * K));
+  chunk -= chunk % DB;
+  const uint32_t nchunks = (cols + chunk - 1) / chunk;
+  const size_t buf = (size_t)K * chunk;
+  uint32_t* T = nullptr;
+  if (const hipError_t ea = hipMallocAsync((void**)&T, 3 * buf * sizeof(uint32_t), s)) return alloc_fail(ea, "decode scratch allocation");
+  int rc = 0;
+  for (uint32_t i = 0; i < nchunks + 2 && !rc; i++) {
+    DecChunks a = base;
+    uint32_t items = 0;
+    if (i < nchunks) {
+      a.tg = T + (i % 3) * buf;
+      a.gc0 = i * chunk;
+      a.gcols = std::min(chunk, cols - a.gc0);
+      items = std::max(items, (K / DB) * (a.gcols / DB));
+    }
+    if (i >= 1 && i <= nchunks) {
+      a.tc = T + ((i - 1) % 3) * buf;
+      a.ccols = std::min(chunk, cols - (i - 1) * chunk);
+      items = std::max(items, a.ccols / (uint32_t)PL::W);
+    }
+    if (i >= 2) {
+      a.ts = T + ((i - 2) % 3) * buf;
+      a.sc0 = (i - 2) * chunk;
+      a.scols = std::min(chunk, cols - a.sc0);
+      items = std::max(items, (K / DB) * (a.scols / DB));
+    }
+    hipLaunchKernelGGL(dec_chunks<PL>, dim3(std::min(grid, items)), dim3(PL::NT), PL::LDS_WORDS * sizeof(int32_t), s,
+                       a, srow, tabs);
+    hip_note(rc, hipGetLastError(), "decode launch");
+  }
+  hip_note(rc, hipFreeAsync(T, s), "hipFreeAsync");
+  return rc;
 }
 
 // data[j * cols + c] = T[c * stride + j] for j < k, except the rows j whose
@@ -427,9 +602,12 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
   int rc = 0;
   hip_note(rc, hipMemcpy(d_blk, host.data(), up_words * 4, hipMemcpyHostToDevice), "decode vectors upload");
   if (!rc) hip_note(rc, hipMemcpy(d_known, known.data(), n, hipMemcpyHostToDevice), "decode vectors upload");
-  // Scratch: the k known rows per column (FP64 path) or all n rows.
+  // Whole 64-column blocks of 16-byte aligned squares take the chunked FP64
+  // path (dec_chunks, its own scratch); otherwise the scratch holds the k
+  // known rows per column (FP64 path) or all n rows.
+  const bool chunked = fp64 && cols % DB == 0 && !(((uintptr_t)d_enc | (uintptr_t)d_data) & 15);
   const uint32_t t_rows = fp64 ? k : n;
-  if (!rc) {
+  if (!rc && !chunked) {
     if (const hipError_t ea = hipMallocAsync((void**)&T, (size_t)cols * t_rows * sizeof(uint32_t), s)) {
       T = nullptr;
       rc = alloc_fail(ea, "decode scratch allocation");
@@ -442,7 +620,14 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
   if (!rc) {
     hipLaunchKernelGGL(dec_prep, dim3((2 * n + 255) / 256), dim3(256), 0, s, d_x, d_known, d_ep, ne, log_n,
                        to_mont_host(DEC_COSET), fp64 ? 1 : 0, to_mont_host(n), d_zk, d_zinv);
-    if (fp64) {
+    if (chunked) {
+      const DecChunks base{d_enc, d_data, cols, k, d_rows, d_zk, d_known, nullptr, 0, 0, nullptr, 0, nullptr, 0, 0};
+      hip_note(rc, hipGetLastError(), "decode launch");
+      if (!rc)
+        rc = with_decode_plan((int)log_n, [&](auto plan) -> int {
+          return decode_chunked<decltype(plan)>(st, base, d_zinv, *dtabs, s);
+        });
+    } else if (fp64) {
       hipLaunchKernelGGL(dec_gather_ranked, dim3((cols + TT - 1) / TT, (k + TT - 1) / TT), dim3(TT, GATHER_NY), 0, s, d_enc,
                          cols, k, d_rows, d_zk, T, d_data);
       (void)with_decode_plan((int)log_n, [&](auto plan) -> int {
@@ -458,9 +643,11 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
       hipLaunchKernelGGL(dec_fused, dim3(cols), dim3(DEC_NT), lds, s, T, log_n, tn.fwd, tn.inv, tk.fwd, d_hpow,
                          d_zinv, d_hinv);
     }
-    hipLaunchKernelGGL(dec_scatter, dim3((k + TT - 1) / TT, (cols + TT - 1) / TT), dim3(TT, SCATTER_NY), 0, s, T, t_rows, k,
-                       cols, fp64 ? d_known : nullptr, d_data);
-    hip_note(rc, hipGetLastError(), "decode launch");
+    if (!chunked) {
+      hipLaunchKernelGGL(dec_scatter, dim3((k + TT - 1) / TT, (cols + TT - 1) / TT), dim3(TT, SCATTER_NY), 0, s, T, t_rows,
+                         k, cols, fp64 ? d_known : nullptr, d_data);
+      hip_note(rc, hipGetLastError(), "decode launch");
+    }
   }
   if (T) hip_note(rc, hipFreeAsync(T, s), "hipFreeAsync");
   // d_blk is read by the kernels just queued: free it once they are done.
