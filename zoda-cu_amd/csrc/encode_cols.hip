@@ -91,6 +91,16 @@ __global__ __launch_bounds__(nttf::NT) void encode_cols_kernel(ColsWork w, const
     for (int q = 0; q < ecol::TB_RUNS; q++) tv[q] = src.load4(tb_lane_in, 4u * q * (nttf::NT / 16) * w.pitch);
   };
   if constexpr (TR) fetch_block(blockIdx.x);
+  if constexpr (TR && COL) {
+    // Stores into a zero-record buffer (they store nothing): entering the
+    // loop, as around it, the block's loads are then followed by a tile's
+    // stores before the next pair loads, so the wait for the block leaves the
+    // previous tile's stores outstanding (the compiler takes the smaller count
+    // where the two paths meet).
+    const GBuf none(w.out, 0u);
+#pragma unroll
+    for (int q = 0; q < ecol::PAIRS; q++) none.store4(U4{0, 0, 0, 0}, lane_out, 16u * q * nttf::NT);
+  }
   for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
     // Each pair stays one 64-bit register pair from its load to the store
     // phase: split earlier, the compiler moves the second value into its

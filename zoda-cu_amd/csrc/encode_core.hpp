@@ -226,6 +226,14 @@ struct GBuf {
   // (the record count is pinned to a scalar register: a descriptor in vector
   // registers turns every access into a loop over its lanes)
   __device__ GBuf(const uint32_t* p, bool live) : r(make(p, live)) {}
+  // `bytes` records: accesses at or past that many bytes from p load 0 and
+  // store nothing (bytes uniform over the wave)
+  __device__ GBuf(const uint32_t* p, uint32_t bytes) : r(make_n(p, bytes)) {}
+  static __device__ __amdgpu_buffer_rsrc_t make_n(const uint32_t* p, uint32_t bytes) {
+    int nr = __builtin_amdgcn_readfirstlane((int)bytes);
+    asm volatile("" : "+s"(nr));
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(p), 0, nr, 0x00020000);
+  }
   static __device__ __amdgpu_buffer_rsrc_t make(const uint32_t* p, bool live) {
     int nr = __builtin_amdgcn_readfirstlane(live ? -1 : 0);
     asm volatile("" : "+s"(nr));

@@ -122,6 +122,16 @@ ZHD U4 tile_chunk(int tid, int q, const TileSpan& ts, const LoadF& load4) {
   return x;
 }
 
+// u32 offset of tile value v from the tile's first poly: poly v >> L, index
+// v mod n.  For v = 4 tid + 4 NT q it splits into a per-thread part
+// (chunk_offset(4 tid)) and a per-q part (chunk_offset(4 NT q)): 4 tid < 4 NT
+// = 1024, so either n <= 1024 and 4 NT q is whole polys, or n > 1024 and
+// neither part leaves its poly.
+template <class PL>
+ZHD uint32_t chunk_offset(uint32_t v, uint32_t stride) {
+  return (v >> PL::L) * stride + (v & (PL::N - 1));
+}
+
 ZHD void put_chunk(int tid, int q, int32_t* __restrict__ lds, const U4& x) {
   int32_t* d = lds + phys(4 * (tid + q * NT));
   d[0] = (int32_t)x.x;

@@ -108,6 +108,13 @@ __global__ void scale_global(uint32_t* __restrict__ data, uint32_t log_n, uint32
 // From n = 512 the next tile's values are loaded into registers while the
 // passes run (sixteen a thread): 3-6 % faster at n = 1024 and 4096 (2^30
 // values, alternating runs), within noise or slower at n = 256.
+// Loads and stores are raw-buffer accesses through a descriptor per tile
+// whose record count ends at its last live poly: the lanes of a ragged last
+// tile past it load zeros and store nothing, with no branch around the
+// access (a branch leaves the compiler's count of outstanding accesses
+// conservative, and each tile then waited for the previous one's stores
+// before its prefetched values could go to LDS).  Every byte offset stays
+// below 2^32 (ntt_fp64_fits).
 template <int L, bool INV>
 __global__ __launch_bounds__(nttf::NT) void ntt_poly_fp64(uint32_t* __restrict__ data, uint32_t polys,
                                                           uint32_t stride, const uint32_t* __restrict__ twg) {
@@ -120,25 +127,39 @@ __global__ __launch_bounds__(nttf::NT) void ntt_poly_fp64(uint32_t* __restrict__
   const uint32_t ntiles = (polys + PL::POLYS - 1) / PL::POLYS;
   constexpr bool PREFETCH = L >= 9;
   constexpr int CH = nttf::TILE / 4 / nttf::NT;
-  auto load = [&](size_t poly, uint32_t j) -> U4 { return *reinterpret_cast<const U4*>(data + poly * stride + j); };
-  U4 pre[PREFETCH ? CH : 1];
-  // the values of tile t (zeros past the last tile) into pre
-  auto fetch = [&](uint32_t t) {
-    const uint32_t first = t * (uint32_t)PL::POLYS;
-    const nttf::TileSpan ts{t < ntiles ? min((uint32_t)PL::POLYS, polys - first) : 0u, (size_t)first};
-#pragma unroll
-    for (int q = 0; q < CH; q++) pre[q] = nttf::tile_chunk<PL>(tid, q, ts, load);
+  const uint32_t lane = 4u * nttf::chunk_offset<PL>(4u * tid, stride);
+  // the tile's buffer: its first poly, records up to the end of its last live one
+  auto tile_buf = [&](uint32_t t) -> GBuf {
+    const uint32_t first = t < ntiles ? t * (uint32_t)PL::POLYS : 0u;
+    const uint32_t live = t < ntiles ? min((uint32_t)PL::POLYS, polys - first) : 0u;
+    return GBuf(data + (size_t)first * stride, live ? 4u * ((live - 1) * stride + (uint32_t)PL::N) : 0u);
   };
-  if constexpr (PREFETCH) fetch(blockIdx.x);
+  auto qoff = [&](int q) -> uint32_t { return 4u * nttf::chunk_offset<PL>(4u * nttf::NT * q, stride); };
+  U4 pre[CH];
+  auto fetch = [&](uint32_t t) {
+    const GBuf src = tile_buf(t);
+#pragma unroll
+    for (int q = 0; q < CH; q++) pre[q] = src.load4(lane, qoff(q));
+  };
+  if constexpr (PREFETCH) {
+    fetch(blockIdx.x);
+    // Stores into a zero-record buffer (they store nothing): entering the
+    // loop, as around it, the prefetch is then followed by CH stores, so the
+    // wait before put_chunk can leave the previous tile's stores outstanding
+    // (the compiler takes the smaller count where the two paths meet).
+    const GBuf none(data, 0u);
+#pragma unroll
+    for (int q = 0; q < CH; q++) none.store4_cached(U4{0, 0, 0, 0}, lane, qoff(q));
+  }
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const uint32_t first = t * (uint32_t)PL::POLYS;
-    const nttf::TileSpan ts{min((uint32_t)PL::POLYS, polys - first), (size_t)first};
+    const GBuf buf = tile_buf(t);
     __syncthreads();  // the tables, or the previous tile's last reads of LDS
     if constexpr (PREFETCH) {
 #pragma unroll
       for (int q = 0; q < CH; q++) nttf::put_chunk(tid, q, lds, pre[q]);
     } else {
-      nttf::load_tile<PL>(tid, ts, lds, load);
+#pragma unroll
+      for (int q = 0; q < CH; q++) nttf::put_chunk(tid, q, lds, buf.load4(lane, qoff(q)));
     }
     __syncthreads();
     if constexpr (PREFETCH) fetch(t + gridDim.x);
@@ -152,9 +173,15 @@ __global__ __launch_bounds__(nttf::NT) void ntt_poly_fp64(uint32_t* __restrict__
       nttf::pass<PL, 2, INV>(tid, lds, tw, k);
       __syncthreads();
     }
-    nttf::store_tile<PL>(tid, ts, lds, [&](size_t poly, uint32_t kq, const U4& v) {
-      store16(data + poly * stride + kq, v);
-    });
+    // the natural-order values of chunk q out of their slots (store_tile's
+    // order), every lane storing
+#pragma unroll
+    for (int q = 0; q < CH; q++) {
+      const int v = 4 * (tid + q * nttf::NT), pb = (v >> PL::L) << PL::L, kq = v & (PL::N - 1);
+      const U4 x{(uint32_t)lds[nttf::phys(pb + PL::slot_of(kq))], (uint32_t)lds[nttf::phys(pb + PL::slot_of(kq + 1))],
+                 (uint32_t)lds[nttf::phys(pb + PL::slot_of(kq + 2))], (uint32_t)lds[nttf::phys(pb + PL::slot_of(kq + 3))]};
+      buf.store4_cached(x, lane, qoff(q));
+    }
   }
 }
 
@@ -205,7 +232,9 @@ static int launch_fp64_size(DeviceState* st, uint32_t* d, uint32_t log_n, uint32
 }
 
 bool ntt_fp64_fits(const uint32_t* d_values, uint32_t log_n, uint32_t stride) {
-  return log_n >= 2 && log_n <= 12 && stride % 4 == 0 && ((uintptr_t)d_values & 15) == 0;
+  // a tile's byte offsets (4096 / n polys of `stride` u32) below 2^32
+  return log_n >= 2 && log_n <= 12 && stride % 4 == 0 && ((uintptr_t)d_values & 15) == 0 &&
+         ((uint64_t)(nttf::TILE >> log_n) * stride * 4u) < (1ull << 32);
 }
 
 int launch_ntt(DeviceState* st, uint32_t* d_values, uint32_t log_n, uint32_t polys, uint32_t stride,
