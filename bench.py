@@ -63,13 +63,14 @@ def parse():
     return args
 
 
-def traffic_from_profile(side: int):
+def traffic_from_profile(side, layout: str = ""):
     """HBM bytes per launch of the encode kernel measured by rocprofv3 --pmc
     (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a
     copy kernel of known size), from the committed summary's
-    `hbm_bytes_per_launch:` line; null when no measurement exists."""
+    `hbm_bytes_per_launch:` line (per encode call for the column-major layout,
+    `layout="col_"`); null when no measurement exists."""
     import re
-    path = os.path.join(ROOT, "profiles", f"encode_{side}_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"encode_{layout}{side}_traffic.md")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -504,7 +505,7 @@ def column_major_leg(zoda_hip, torch, t_in, steps: int):
                     "square, device-resident",
         "ms_avg": round(avg, 4), "ms_p50": round(p50, 4),
         "gbps": round(side * side * SHARE_BYTES * 8 / 1e9 / (avg / 1e3), 2),
-        "roofline": roofline_of(12 * side * cols, avg),
+        "roofline": roofline_of(12 * side * cols, avg, traffic_from_profile(side, "col_")),
         "spot_check": ok,
     }
 
