@@ -303,9 +303,10 @@ __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_chunks(DecChunks a, cons
                                                                DecodeTabs tabs) {
   static_assert(DB * DB_PITCH <= PL::TILE_WORDS, "a transpose block fits the tile's LDS");
   constexpr uint32_t K = PL::S / 2, KB = K / DB;  // rank (or data-row) blocks per column band
+  constexpr int GI = (DB * DB / 4 + PL::NT - 1) / PL::NT;  // 16-byte runs of a block per thread
   extern __shared__ int32_t lds[];
   uint32_t* sh = reinterpret_cast<uint32_t*>(lds);
-  __shared__ uint32_t rid[DB], zr[DB];
+  __shared__ uint32_t rid[2][DB], zr[2][DB];  // a gather block's row numbers and Z(w^r), two items
   const int tid = (int)threadIdx.x;
   dec_load_tables<PL>(tid, lds, tabs);
   DecodeTabs ct = tabs;  // the roots from constant memory
@@ -314,21 +315,53 @@ __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_chunks(DecChunks a, cons
   const uint32_t ntile = a.tc ? a.ccols / PL::W : 0u;
   const uint32_t ns = a.ts ? KB * (a.scols / DB) : 0u;
   const uint32_t nitems = max(ng, max(ntile, ns));
-  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+  // Item b's gather and scatter loads are issued one item ahead (before the
+  // passes of the column tile in between), so their latency hides behind
+  // arithmetic; an item past the end of its kind loads harmlessly (the first
+  // row of the square; a zero-record buffer) instead of branching.
+  U4 gv[GI], sv[GI];
+  auto stage = [&](uint32_t b, int buf) {  // the block's row numbers into LDS
+    if (tid < DB) {
+      const bool live = b < ng;
+      const uint32_t r = live ? a.rows[(b % KB) * DB + tid] : 0u;
+      rid[buf][tid] = r;
+      zr[buf][tid] = live ? a.zk[r] : 0u;
+    }
+  };
+  auto fetch = [&](uint32_t b, int buf) {  // after a barrier that follows stage(b, buf)
+    const bool gl = b < ng;
+    const size_t gcol = gl ? (size_t)a.gc0 + (b / KB) * DB : 0u;
+#pragma unroll
+    for (int g = 0; g < GI; g++) {
+      const int idx = tid + g * PL::NT;
+      const uint32_t q = (idx / 16) % DB, c4 = gl ? 4 * (idx % 16) : 0u;
+      gv[g] = *reinterpret_cast<const U4*>(a.enc + (size_t)rid[buf][q] * a.cols + gcol + c4);
+    }
+    const GBuf src(a.ts, b < ns);
+    const uint32_t j0 = (b % KB) * DB, cb = (b / KB) * DB;
+#pragma unroll
+    for (int g = 0; g < GI; g++) {
+      const int idx = tid + g * PL::NT;
+      const uint32_t c = (idx / 16) % DB, j4 = 4 * (idx % 16);
+      sv[g] = src.load4(4u * ((cb + c) * K + j0 + j4), 0u);
+    }
+  };
+  stage(blockIdx.x, 0);
+  __syncthreads();
+  fetch(blockIdx.x, 0);
+  int buf = 0;
+  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x, buf ^= 1) {
+    const uint32_t next = it + gridDim.x;
     if (it < ng) {  // gather: ranks i0.., columns cb.. of chunk g
       const uint32_t i0 = (it % KB) * DB, cb = (it / KB) * DB;
       __syncthreads();  // the previous item's last reads of LDS
-      if (tid < DB) {
-        const uint32_t r = a.rows[i0 + tid];
-        rid[tid] = r;
-        zr[tid] = a.zk[r];
-      }
-      __syncthreads();
-      for (int idx = tid; idx < DB * DB / 4; idx += PL::NT) {
-        const uint32_t q = idx / 16, c4 = 4 * (idx % 16), r = rid[q], z = zr[q];
-        const size_t col = (size_t)a.gc0 + cb + c4;
-        const U4 e = *reinterpret_cast<const U4*>(a.enc + (size_t)r * a.cols + col);
-        if (!(r & 1u)) store16(a.data + (size_t)(r >> 1) * a.cols + col, e);  // a known data row
+#pragma unroll
+      for (int g = 0; g < GI; g++) {
+        const int idx = tid + g * PL::NT;
+        if (idx >= DB * DB / 4) break;
+        const uint32_t q = idx / 16, c4 = 4 * (idx % 16), r = rid[buf][q], z = zr[buf][q];
+        const U4 e = gv[g];
+        if (!(r & 1u)) store16(a.data + (size_t)(r >> 1) * a.cols + a.gc0 + cb + c4, e);  // a known data row
         uint32_t* d = sh + q * DB_PITCH + c4;
         d[0] = mont_mul(e.x, z);
         d[1] = mont_mul(e.y, z);
@@ -342,12 +375,36 @@ __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_chunks(DecChunks a, cons
         store16(a.tg + (size_t)(cb + c) * K + i0 + q4, U4{s[0], s[DB_PITCH], s[2 * DB_PITCH], s[3 * DB_PITCH]});
       }
     }
+    if (it < ns) {  // scatter: data rows j0.., columns cb.. of chunk s (the erased ones)
+      const uint32_t j0 = (it % KB) * DB, cb = (it / KB) * DB;
+      __syncthreads();
+#pragma unroll
+      for (int g = 0; g < GI; g++) {
+        const int idx = tid + g * PL::NT;
+        if (idx >= DB * DB / 4) break;
+        const uint32_t c = idx / 16, j4 = 4 * (idx % 16);
+        uint32_t* d = sh + j4 * DB_PITCH + c;
+        d[0] = sv[g].x;
+        d[DB_PITCH] = sv[g].y;
+        d[2 * DB_PITCH] = sv[g].z;
+        d[3 * DB_PITCH] = sv[g].w;
+      }
+      __syncthreads();
+      for (int idx = tid; idx < DB * DB / 4; idx += PL::NT) {
+        const uint32_t j = idx / 16, c4 = 4 * (idx % 16);
+        if (a.known[2 * (j0 + j)]) continue;  // written by the gather
+        const uint32_t* s = sh + j * DB_PITCH + c4;
+        store16(a.data + (size_t)(j0 + j) * a.cols + a.sc0 + cb + c4, U4{s[0], s[1], s[2], s[3]});
+      }
+    }
+    stage(next, buf ^ 1);  // read after the next barrier
     if (it < ntile) {  // the column kernel on tile it of chunk c
       int tid2 = tid;
       asm("" : "+v"(tid2) : "s"(it));  // keep the per-thread address arithmetic in the loop
       __syncthreads();  // the tables, or the previous reads of LDS
       dec_load<PL>(tid2, it, a.ccols, a.tc, a.rows, lds);
       __syncthreads();
+      fetch(next, buf ^ 1);  // arrives during the passes
       dec_step<PL, 0>(tid2, lds, ct, srow);
       __syncthreads();
       dec_step<PL, 1>(tid2, lds, ct, srow);
@@ -359,26 +416,9 @@ __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_chunks(DecChunks a, cons
       dec_step<PL, 4>(tid2, lds, ct, srow);
       __syncthreads();
       dec_store<PL>(tid2, it, a.ccols, a.tc, lds);
-    }
-    if (it < ns) {  // scatter: data rows j0.., columns cb.. of chunk s (the erased ones)
-      const uint32_t j0 = (it % KB) * DB, cb = (it / KB) * DB;
+    } else {
       __syncthreads();
-      for (int idx = tid; idx < DB * DB / 4; idx += PL::NT) {
-        const uint32_t c = idx / 16, j4 = 4 * (idx % 16);
-        const U4 v = *reinterpret_cast<const U4*>(a.ts + (size_t)(cb + c) * K + j0 + j4);
-        uint32_t* d = sh + j4 * DB_PITCH + c;
-        d[0] = v.x;
-        d[DB_PITCH] = v.y;
-        d[2 * DB_PITCH] = v.z;
-        d[3 * DB_PITCH] = v.w;
-      }
-      __syncthreads();
-      for (int idx = tid; idx < DB * DB / 4; idx += PL::NT) {
-        const uint32_t j = idx / 16, c4 = 4 * (idx % 16);
-        if (a.known[2 * (j0 + j)]) continue;  // written by the gather
-        const uint32_t* s = sh + j * DB_PITCH + c4;
-        store16(a.data + (size_t)(j0 + j) * a.cols + a.sc0 + cb + c4, U4{s[0], s[1], s[2], s[3]});
-      }
+      fetch(next, buf ^ 1);
     }
   }
 }
