@@ -193,6 +193,41 @@ def test_column_kernel_matches_row_layout(zh, torch, side, cols):
     assert np.array_equal(to_host(col[idx, :].t().contiguous()), O.encode(data, threads=4))
 
 
+def test_column_layout_under_graph_capture(zh, torch):
+    # captured into a graph, the column-major encode of whole 64-column blocks
+    # leaves the chunked path (its scratch comes from the stream-ordered
+    # allocator) for the one-kernel form; three replays, new inputs and eager
+    # work in between, each against the eager (chunked) encode
+    side, cols = 2048, 4096
+    s = torch.cuda.current_stream().cuda_stream
+    t_in = torch.empty((side, cols), dtype=torch.int32, device="cuda")
+    t_out = torch.empty((cols, 2 * side), dtype=torch.int32, device="cuda")
+    ref = torch.empty_like(t_out)
+    zh.fill_uniform(t_in, seed=1)
+    zh.encode_device(t_in.data_ptr(), side, cols, side, ref.data_ptr(), 1, s)
+    part = torch.empty((cols - 16, 2 * side), dtype=torch.int32, device="cuda")
+    narrow = t_in[:, :cols - 16].contiguous()  # the one-kernel form's set-up, eagerly
+    zh.encode_device(narrow.data_ptr(), side, cols - 16, side, part.data_ptr(), 1, s)
+    torch.cuda.synchronize()
+    print_case("col-graph", side=side, cols=cols)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        zh.encode_device(t_in.data_ptr(), side, cols, side, t_out.data_ptr(), 1,
+                         torch.cuda.current_stream().cuda_stream)
+    for rep in range(3):
+        zh.fill_uniform(t_in, seed=10 + rep)
+        t_out.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        zh.encode_device(t_in.data_ptr(), side, cols, side, ref.data_ptr(), 1, s)
+        torch.cuda.synchronize()
+        assert torch.equal(t_out, ref), rep
+    data = to_host(t_in[:, [0, cols - 1]].contiguous())
+    assert np.array_equal(to_host(t_out[[0, cols - 1], :].t().contiguous()), O.encode(data, threads=2))
+    del g
+
+
 def test_column_major_slab_2048(zh, torch):
     # the exact slab zoda_hip_encode_host encodes at 2048^2 (4096 columns of
     # 2048 rows, whole tiles, column-major output), every value checked
