@@ -57,9 +57,18 @@ struct Plan {
   // [j][group >> LO] (a group's index within its column: see group_slot).
   // Lanes run through consecutive groups, so every table read of a wave is
   // consecutive words (no LDS bank conflicts).
+  // At side 4096 the first INTT pass's [k][low] table (4096 entries) is
+  // factored, w^(low k) = w^(lh 2^FA k) w^(ll k) with low = lh 2^FA + ll, into
+  // [k][lh] and [k][ll] (256 entries each), at one more multiplication in
+  // that pass: the tables then leave LDS for three work-groups per CU.
+  static constexpr bool FT0 = L >= 12 && P > 2;
+  static constexpr int FA = LO(0) / 2;
+  static constexpr int INTT_WORDS(int i) {
+    return (i == 0 && FT0) ? (1 << (R(0) + LO(0) - FA)) + (1 << (R(0) + FA)) : 1 << (LO(i) + R(i));
+  }
   static constexpr int INTT_OFF(int i) {
     int o = 0;
-    for (int j = 0; j < i; j++) o += 1 << (LO(j) + R(j));
+    for (int j = 0; j < i; j++) o += INTT_WORDS(j);
     return o;
   }
   static constexpr int TWIST_OFF = INTT_OFF(P - 2);  // last INTT pass, S entries
@@ -92,10 +101,21 @@ constexpr uint32_t freq_of_slot(uint32_t slot, int upto) {
 template <class PL>
 inline void build_col_tables(uint32_t* tw) {
   const uint32_t s_inv = inv_mod_host((uint32_t)PL::S), w2s = root_of_unity(PL::L + 1);
-  // INTT passes i < P - 2: [k][low] = w_(2^(LO+R))^-(low k)
+  // INTT passes i < P - 2: [k][low] = w_(2^(LO+R))^-(low k); factored (FT0):
+  // [k][lh] = w^-(lh 2^FA k), then [k][ll] = w^-(ll k)
   for (int i = 0; i + 2 < PL::P; i++) {
     const uint32_t wi = inv_mod_host(root_of_unity(PL::LO(i) + PL::R(i)));
     const int lows = 1 << PL::LO(i), ks = 1 << PL::R(i);
+    if (i == 0 && PL::FT0) {
+      const int nlh = 1 << (PL::LO(0) - PL::FA), nll = 1 << PL::FA;
+      for (int k = 0; k < ks; k++) {
+        for (int lh = 0; lh < nlh; lh++)
+          tw[PL::INTT_OFF(0) + k * nlh + lh] = pow_mod_host(wi, (uint64_t)lh * nll * k);
+        for (int ll = 0; ll < nll; ll++)
+          tw[PL::INTT_OFF(0) + ks * nlh + k * nll + ll] = pow_mod_host(wi, (uint64_t)ll * k);
+      }
+      continue;
+    }
     for (int k = 0; k < ks; k++)
       for (int low = 0; low < lows; low++)
         tw[PL::INTT_OFF(i) + k * lows + low] = pow_mod_host(wi, (uint64_t)low * k);
@@ -172,6 +192,18 @@ ZHD void intt_pass(int tid, int32_t* __restrict__ lds, const uint32_t* __restric
       const int gl = (tid + q * NT) & ((1 << (PL::L - R)) - 1);  // group within its column
       slice::mul_mem_batch(x, [&](int kk) -> slice::Tw2 {
         const double w = (double)tw[PL::TWIST_OFF + (kk << (PL::L - R)) + gl];
+        return slice::Tw2{w, w * PINV};
+      }, r, k);
+    } else if constexpr (I == 0 && PL::FT0) {
+      constexpr int NLH = 1 << (LO - PL::FA), NLL = 1 << PL::FA;
+      const int lh = low >> PL::FA, ll = low & (NLL - 1);
+#pragma unroll
+      for (int kk = 1; kk < RR; kk++) {
+        const double w = (double)tw[PL::INTT_OFF(0) + kk * NLH + lh];
+        x[kk] = slice::mul_reg(x[kk], w, w * PINV, k);
+      }
+      slice::mul_mem_batch(x, [&](int kk) -> slice::Tw2 {
+        const double w = (double)tw[PL::INTT_OFF(0) + RR * NLH + kk * NLL + ll];
         return slice::Tw2{w, w * PINV};
       }, r, k);
     } else {
