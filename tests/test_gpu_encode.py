@@ -172,8 +172,8 @@ def test_slice_kernel_tile_counts_4096(zh, torch, tiles):
     assert np.array_equal(gpu_encode(zh, torch, data), O.encode(data, threads=16))
 
 
-@pytest.mark.parametrize("side,cols", [(2048, 64), (2048, 64 * 1000), (2048, 128 * 2048),
-                                       (4096, 64), (4096, 64 * 300)])
+@pytest.mark.parametrize("side,cols", [(256, 64 * 1100), (512, 64 * 37), (1024, 64 * 300), (2048, 64),
+                                       (2048, 64 * 1000), (2048, 128 * 2048), (4096, 64), (4096, 64 * 300)])
 def test_column_kernel_matches_row_layout(zh, torch, side, cols):
     # whole squares with cols a multiple of 64 go into the column-major layout
     # by transpose + column kernel: fewer tiles than work-groups, several per
