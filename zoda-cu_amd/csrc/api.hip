@@ -61,6 +61,32 @@ int get_buffer(DeviceState* st, int slot, size_t bytes, void** out) {
   return 0;
 }
 
+int pool_alloc(DeviceState* st, size_t bytes, hipStream_t stream, void** out) {
+  {
+    std::lock_guard<std::recursive_mutex> lock(st->mu);
+    if (!st->pool) {
+      hipMemPoolProps props{};
+      props.allocType = hipMemAllocationTypePinned;
+      props.handleTypes = hipMemHandleTypeNone;
+      props.location.type = hipMemLocationTypeDevice;
+      props.location.id = st->device;
+      hipMemPool_t pool = nullptr;
+      ZODA_HIP(hipMemPoolCreate(&pool, &props));
+      uint64_t keep = POOL_KEEP_BYTES;
+      if (const hipError_t e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep)) {
+        (void)hipMemPoolDestroy(pool);
+        return fail((int)e, "setting the scratch pool's release threshold failed: %s", hipGetErrorString(e));
+      }
+      st->pool = pool;
+    }
+  }
+  if (const hipError_t ea = hipMallocFromPoolAsync(out, bytes, st->pool, stream)) {
+    *out = nullptr;
+    return alloc_fail(ea, "scratch allocation");
+  }
+  return 0;
+}
+
 int get_streams(DeviceState* st, int count, hipStream_t** out) {
   while ((int)st->streams.size() < count) {
     hipStream_t s;

@@ -114,12 +114,21 @@ struct DeviceState {
   // log n << 1 | inverse; whether a size's caller roots are the standard ones.
   std::map<uint32_t, uint32_t*> ntt_fp64;
   std::map<uint32_t, bool> user_standard;
+  // The library's own stream-ordered pool for kernel scratch (pool_alloc).
+  hipMemPool_t pool = nullptr;
 };
 
 // State of the calling thread's current device (created on first use).
 int current_state(DeviceState** out);
 int get_buffer(DeviceState* st, int slot, size_t bytes, void** out);
 int get_streams(DeviceState* st, int count, hipStream_t** out);
+// Stream-ordered scratch from the library's pool (freed with hipFreeAsync on
+// the same stream).  The pool keeps up to POOL_KEEP_BYTES between calls, so a
+// caller that synchronizes after every call does not pay an allocation from
+// the system each time (the default pool returns everything at each
+// synchronization).
+constexpr uint64_t POOL_KEEP_BYTES = 1ull << 30;
+int pool_alloc(DeviceState* st, size_t bytes, hipStream_t stream, void** out);
 
 // Standard-root Montgomery tables for size 2^log_n.
 int get_mont_tables(DeviceState* st, uint32_t log_n, MontTables* out);

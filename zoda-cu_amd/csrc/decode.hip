@@ -501,7 +501,7 @@ int decode_chunked(DeviceState* st, DecChunks base, const uint32_t* srow, const 
   const uint32_t nchunks = (cols + chunk - 1) / chunk;
   const size_t buf = (size_t)K * chunk;
   uint32_t* T = nullptr;
-  if (const hipError_t ea = hipMallocAsync((void**)&T, 3 * buf * sizeof(uint32_t), s)) return alloc_fail(ea, "decode scratch allocation");
+  ZODA_TRY(pool_alloc(st, 3 * buf * sizeof(uint32_t), s, (void**)&T));
   int rc = 0;
   for (uint32_t i = 0; i < nchunks + 2 && !rc; i++) {
     DecChunks a = base;
@@ -648,10 +648,7 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
   const bool chunked = fp64 && cols % DB == 0 && !(((uintptr_t)d_enc | (uintptr_t)d_data) & 15);
   const uint32_t t_rows = fp64 ? k : n;
   if (!rc && !chunked) {
-    if (const hipError_t ea = hipMallocAsync((void**)&T, (size_t)cols * t_rows * sizeof(uint32_t), s)) {
-      T = nullptr;
-      rc = alloc_fail(ea, "decode scratch allocation");
-    }
+    rc = pool_alloc(st, (size_t)cols * t_rows * sizeof(uint32_t), s, (void**)&T);
   }
   const size_t lds = (size_t)n * sizeof(uint32_t);
   if (!rc && !fp64 && lds > 65536)

@@ -228,7 +228,7 @@ static int encode_general(DeviceState* st, const uint32_t* d_in, uint32_t k, uin
   const uint32_t n_kn = 1u << log_nkn;
   const size_t total = (size_t)cols * n_kn;
   uint32_t* tmp = d_out;
-  if (layout == ZODA_LAYOUT_ROW) ZODA_HIP(hipMallocAsync((void**)&tmp, total * 4, stream));
+  if (layout == ZODA_LAYOUT_ROW) ZODA_TRY(pool_alloc(st, total * 4, stream, (void**)&tmp));
   const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 1u << 16);
   hipLaunchKernelGGL(gather_columns, dim3(blocks), dim3(256), 0, stream, d_in, k, cols, n_kn, tmp);
   ZODA_HIP(hipGetLastError());

@@ -19,8 +19,8 @@
 // Cache, and the chunks reuse it: the transposed values go to the cache and
 // come back from it instead of making a round trip to HBM.
 //
-// The scratch comes from the stream-ordered allocator, so encodes on
-// different streams never share it.
+// The scratch comes from the library's stream-ordered pool (pool_alloc), so
+// encodes on different streams never share it.
 #include "common.hpp"
 #include "encode_cols_core.hpp"
 
@@ -177,7 +177,7 @@ static int launch_cols_plan(DeviceState* st, const uint32_t* d_in, uint32_t cols
   const uint32_t nchunks = (cols + chunk - 1) / chunk;
   uint32_t* scratch = nullptr;
   const size_t buf = (size_t)k * chunk;  // u32 per buffer
-  ZODA_HIP(hipMallocAsync((void**)&scratch, 2 * buf * sizeof(uint32_t), stream));
+  ZODA_TRY(pool_alloc(st, 2 * buf * sizeof(uint32_t), stream, (void**)&scratch));
   hipError_t e = hipSuccess;
   // launch i: encode chunk i - 1 (buffer (i - 1) % 2), transpose chunk i (buffer i % 2)
   for (uint32_t i = 0; i <= nchunks && e == hipSuccess; i++) {

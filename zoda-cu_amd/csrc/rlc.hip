@@ -311,7 +311,7 @@ extern "C" int zoda_hip_rlc_coefficients(const char* data_root, uint32_t root_le
     return rc;
   }
   uint8_t* d_root = nullptr;
-  ZODA_HIP(hipMallocAsync((void**)&d_root, root_len ? root_len : 1, s));
+  ZODA_TRY(pool_alloc(st, root_len ? root_len : 1, s, (void**)&d_root));
   int rc = 0;
   if (root_len) hip_note(rc, hipMemcpyAsync(d_root, data_root, root_len, hipMemcpyHostToDevice, s), "copying the data root");
   if (!rc) {
@@ -335,7 +335,7 @@ extern "C" int zoda_hip_rlc_rows(const uint32_t* d_square, uint32_t rows, uint32
   ZODA_TRY(current_state(&st));
   hipStream_t s = (hipStream_t)stream;
   uint32_t* partial = nullptr;
-  ZODA_HIP(hipMallocAsync((void**)&partial, (size_t)count * nblk * 4 * sizeof(uint32_t), s));
+  ZODA_TRY(pool_alloc(st, (size_t)count * nblk * 4 * sizeof(uint32_t), s, (void**)&partial));
   int rc = launch_rlc_rows(d_square, rows, cols, d_row_index, count, d_coeffs, d_out, partial, nblk, nrb, s);
   hip_note(rc, hipFreeAsync(partial, s), "hipFreeAsync");
   return rc;
