@@ -33,7 +33,8 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed steps right before the timed ones (the clock settles within about 12)")
     ap.add_argument("--side", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=2048)
     ap.add_argument("--input", choices=("uniform", "ramp"), default="uniform",
@@ -41,7 +42,8 @@ def parse():
                          "(row*cols+col) mod p (src/benchmarks/utils.rs:81-84)")
     ap.add_argument("--no-e2e", action="store_true", help="skip the pinned-host end-to-end timing")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-cols", type=int, default=16384, help="columns in the CPU sample")
+    ap.add_argument("--cpu-cols", type=int, default=65536,
+                    help="columns in the CPU sample (about 20 s of CPU-thread work on 16 threads)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 96-square batched comparison")
     ap.add_argument("--no-served", action="store_true", help="skip the served-trace comparison")
     ap.add_argument("--served-requests", type=int, default=3000,
@@ -82,7 +84,7 @@ def cpu_baseline(side: int, ncols: int, seed: int):
     """The oracle (a C restatement of src/ntt/cpu.rs and the encode of
     src/benchmarks/utils.rs) on `ncols` columns of the same square over the
     process's CPU share (at most 16 threads: one GPU's share of the host),
-    plus a quarter of the sample on one core, the form of the reference's own
+    plus a sixteenth of the sample on one core, the form of the reference's own
     single-threaded CPU encode (src/zoda/babybear.rs:41-44)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -365,7 +367,7 @@ def decode_leg(zoda_hip, torch, t_in, t_out, seed: int, reps: int = 3):
     }
 
 
-def time_events(torch, fn, steps: int, warmup: int = 1):
+def time_events(torch, fn, steps: int, warmup: int = 10):
     """HIP-event time of each of `steps` calls of fn on the current stream
     (after `warmup` untimed calls): (avg ms, p50 ms)."""
     stream = torch.cuda.current_stream()
@@ -446,8 +448,8 @@ def dropin_device_leg(zoda_hip, torch, t_in, t_out, steps: int = 5):
     """The same window on device memory: zoda_hip_ntt_device inverse (n_k,
     stride n_kn) then forward (n_kn, stride n_kn) over the column-major square
     held in HBM -- the drop-in transform kernel itself, without PCIe.  HIP
-    events on the launch stream, average of `steps` after one warm-up whose
-    result is checked against the device encode.  Algorithmic bytes: 8 per
+    events on the launch stream, average of `steps` after one run whose
+    result is checked against the device encode and five untimed ones.  Algorithmic bytes: 8 per
     transformed value, 8 (n_k + n_kn) cols per window."""
     import numpy as np
     side, cols = t_in.shape
@@ -465,6 +467,8 @@ def dropin_device_leg(zoda_hip, torch, t_in, t_out, steps: int = 5):
     torch.cuda.synchronize()
     pick = [0, 1, cols // 3, cols - 1]
     ok = all(np.array_equal(sq[c].cpu().numpy(), t_out[:, c].cpu().numpy()) for c in pick)
+    for _ in range(5):  # untimed, after the check's idle gap (the clock settles)
+        xform()
     st = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     en = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     for i in range(steps):  # in place: later runs transform the previous result (same work)
@@ -630,10 +634,12 @@ def main():
     def step():
         zoda_hip.encode_device(t_in.data_ptr(), side, cols, side, t_out.data_ptr(), 0, sptr)
 
-    for _ in range(args.warmup):
-        step()
+    # spot check of one encode's output (full parity lives in tests/), then
+    # the warm-up steps right before the timed ones: after an idle gap the
+    # GPU's clock takes about a dozen 2048^2 encodes to settle (2.27 ms for
+    # the first, 2.02 ms from the twelfth on)
+    step()
     torch.cuda.synchronize()
-    # spot check of the timed output before timing (full parity lives in tests/)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     pick = [0, 1, cols // 3, cols - 1]
@@ -643,6 +649,8 @@ def main():
     if not np.array_equal(o_s, oracle.encode(d_s)):
         print("bench: output mismatch against the oracle", file=sys.stderr)
         sys.exit(1)
+    for _ in range(args.warmup):
+        step()
 
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
