@@ -493,7 +493,8 @@ extern "C" void zoda_sim_rlc_coefficient(const uint8_t* root, uint32_t len, uint
 
 template <class PL, int STEP>
 static void dec_run_steps(int32_t* lds, const DecodeTabs& T, const uint32_t* srow) {
-  for (int tid = 0; tid < PL::NT; tid++) dec_step<PL, STEP>(tid, lds, T, srow);
+  const slice::MulK mk = slice::make_mulk();
+  for (int tid = 0; tid < PL::NT; tid++) dec_step<PL, STEP>(tid, lds, T, srow, mk);
   if constexpr (STEP < 4) dec_run_steps<PL, STEP + 1>(lds, T, srow);
 }
 
@@ -557,8 +558,7 @@ extern "C" int zoda_sim_decode(const uint32_t* enc, uint32_t k, uint32_t cols, c
     using PL = decltype(plan);
     const DecodeTableImage im = build_decode_tables<PL>();
     if (im.packed.empty()) return -3;
-    static const DecodeConst uniform = make_decode_const();  // the kernel's constant-memory roots
-    const DecodeTabs tabs = make_decode_tabs(im, im.packed.data(), uniform);
+    const DecodeTabs tabs = make_decode_tabs(im, im.packed.data());
     std::vector<int32_t> lds(PL::LDS_WORDS);
     for (int tid = 0; tid < PL::NT; tid++) dec_load_tables<PL>(tid, lds.data(), tabs);
     const uint32_t ntiles = (cols + PL::W - 1) / PL::W;

@@ -232,8 +232,6 @@ __global__ __launch_bounds__(DEC_NT) void dec_fused(uint32_t* __restrict__ T, ui
   for (uint32_t j = threadIdx.x; j < k; j += DEC_NT) col[j] = v[brev_n(j, log_n - 1)];
 }
 
-__constant__ DecodeConst kDecodeConst = make_decode_const();
-
 // Persistent form for 2^8 <= n <= 2^14: tiles of W columns of the
 // rank-packed scratch (k values each) through the FP64 transform pair in LDS
 // (decode_core.hpp); the k even-row values of each column (the data on the
@@ -244,8 +242,7 @@ __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_columns(uint32_t* __rest
                                                       const uint32_t* __restrict__ srow, DecodeTabs tabs) {
   extern __shared__ int32_t lds[];
   dec_load_tables<PL>((int)threadIdx.x, lds, tabs);
-  DecodeTabs ct = tabs;  // the roots from constant memory
-  ct.roots = kDecodeConst.roots;
+  const slice::MulK mk = slice::make_mulk();
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     // A thread's addresses are the same for every tile.  Making its index
     // depend on the tile (through an asm the optimizer cannot see through,
@@ -256,15 +253,15 @@ __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_columns(uint32_t* __rest
     asm("" : "+v"(tid) : "s"(tile));
     dec_load<PL>(tid, tile, cols, T, rows, lds);
     __syncthreads();  // (on the first tile also the tables)
-    dec_step<PL, 0>(tid, lds, ct, srow);
+    dec_step<PL, 0>(tid, lds, tabs, srow, mk);
     __syncthreads();
-    dec_step<PL, 1>(tid, lds, ct, srow);
+    dec_step<PL, 1>(tid, lds, tabs, srow, mk);
     __syncthreads();
-    dec_step<PL, 2>(tid, lds, ct, srow);
+    dec_step<PL, 2>(tid, lds, tabs, srow, mk);
     __syncthreads();
-    dec_step<PL, 3>(tid, lds, ct, srow);
+    dec_step<PL, 3>(tid, lds, tabs, srow, mk);
     __syncthreads();
-    dec_step<PL, 4>(tid, lds, ct, srow);
+    dec_step<PL, 4>(tid, lds, tabs, srow, mk);
     __syncthreads();
     dec_store<PL>(tid, tile, cols, T, lds);
     __syncthreads();  // the next tile's load overwrites LDS
@@ -309,8 +306,7 @@ __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_chunks(DecChunks a, cons
   __shared__ uint32_t rid[2][DB], zr[2][DB];  // a gather block's row numbers and Z(w^r), two items
   const int tid = (int)threadIdx.x;
   dec_load_tables<PL>(tid, lds, tabs);
-  DecodeTabs ct = tabs;  // the roots from constant memory
-  ct.roots = kDecodeConst.roots;
+  const slice::MulK mk = slice::make_mulk();
   const uint32_t ng = a.tg ? KB * (a.gcols / DB) : 0u;
   const uint32_t ntile = a.tc ? a.ccols / PL::W : 0u;
   const uint32_t ns = a.ts ? KB * (a.scols / DB) : 0u;
@@ -405,15 +401,15 @@ __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_chunks(DecChunks a, cons
       dec_load<PL>(tid2, it, a.ccols, a.tc, a.rows, lds);
       __syncthreads();
       fetch(next, buf ^ 1);  // arrives during the passes
-      dec_step<PL, 0>(tid2, lds, ct, srow);
+      dec_step<PL, 0>(tid2, lds, tabs, srow, mk);
       __syncthreads();
-      dec_step<PL, 1>(tid2, lds, ct, srow);
+      dec_step<PL, 1>(tid2, lds, tabs, srow, mk);
       __syncthreads();
-      dec_step<PL, 2>(tid2, lds, ct, srow);
+      dec_step<PL, 2>(tid2, lds, tabs, srow, mk);
       __syncthreads();
-      dec_step<PL, 3>(tid2, lds, ct, srow);
+      dec_step<PL, 3>(tid2, lds, tabs, srow, mk);
       __syncthreads();
-      dec_step<PL, 4>(tid2, lds, ct, srow);
+      dec_step<PL, 4>(tid2, lds, tabs, srow, mk);
       __syncthreads();
       dec_store<PL>(tid2, it, a.ccols, a.tc, lds);
     } else {
@@ -441,8 +437,7 @@ int get_decode_tables(DeviceState* st, const DecodeTabs** out) {
       (void)hipFree(set.packed);
       return fail((int)e, "copying decoder tables failed: %s", hipGetErrorString(e));
     }
-    set.tabs = new DecodeTabs(make_decode_tabs(im, set.packed, DecodeConst{}));
-    set.tabs->roots = nullptr;  // the kernel reads its constant-memory copy (kDecodeConst)
+    set.tabs = new DecodeTabs(make_decode_tabs(im, set.packed));
     it = st->decode.emplace((uint32_t)L, set).first;
   }
   *out = it->second.tabs;

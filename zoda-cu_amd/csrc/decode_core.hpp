@@ -26,8 +26,9 @@
 //
 // Bounds: LDS holds |v| < 2^31 (canonical inputs, then values reduced below
 // 2^30 by the constant multiplication or reduction that ends every pass); a
-// register DFT of at most 32 points keeps them below 2^36, inside mulmod's
-// 2^45, and the product by f < 2^14 below 2^50, inside reduce's 2^51.
+// register DFT of at most 32 points keeps them below 2^36, inside the 2^45
+// of the products (slice::mul_reg, slice::mul_mem), and the product by
+// f < 2^14 below 2^50, inside reduce's 2^51.
 #pragma once
 
 #include <stdint.h>
@@ -36,6 +37,7 @@
 
 #include "encode_core.hpp"
 #include "encode_tables.hpp"
+#include "slice_core.hpp"
 
 namespace zoda {
 
@@ -43,7 +45,6 @@ struct DecodeTabs {
   // canonical u32, copied into LDS: inv1 | fwd1 | fwd2 | inv2
   const uint32_t* packed;
   uint32_t o_inv1, o_fwd1, o_fwd2, o_inv2, packed_words;
-  const Tw* roots;  // w_64^e for e < 64, then w_64^-e (constant memory on the GPU)
 };
 
 template <int L_, int W_, int NT_, int R1_, int R2_, int RD_, int MINW_ = 4>
@@ -77,31 +78,82 @@ ZHD uint32_t lds_at(uint32_t row, uint32_t c) {
   return row * PL::W + c + (row >> PL::RD);
 }
 
-// x * w^(+-low k) for the top digit's table(s) `t` (see DecodePlan::FT).
+// A u32 twiddle from an LDS table as the (w, w/p) pair of the multiplication.
+ZHD slice::Tw2 dec_tw(const uint32_t* __restrict__ t, uint32_t i) {
+  const double w = (double)t[i];
+  return slice::Tw2{w, w * PINV};
+}
+
+// Natural-order DFT of 2^R values in place with the roots w_(2^R)^(+-1) (INV
+// selects the inverse roots), radix-2 decimation in frequency, the
+// bit-reversed result renamed to natural order; |x| < B on entry gives
+// |X| < 2^R B.  Each product is slice::mul_reg's four instructions on the
+// compile-time roots.  Unlike slice::dft it does not batch the products:
+// with four waves per SIMD the other waves hide their latency, and batches
+// would need registers these kernels do not have.
+template <int R, bool INV>
+ZHD void dec_dft(double (&x)[1 << R], const slice::MulK& mk) {
+  constexpr int N = 1 << R;
+  constexpr int D = INV ? 1 : 0;
+#pragma unroll
+  for (int hs = R - 1; hs >= 0; hs--) {
+    const int half = 1 << hs;
+#pragma unroll
+    for (int b = 0; b < N; b += 2 * half) {
+#pragma unroll
+      for (int j = 0; j < half; j++) {
+        const double u = x[b + j], v = x[b + j + half];
+        x[b + j] = u + v;
+        const double d = u - v;
+        x[b + j + half] =
+            j == 0 ? d : slice::mul_reg(d, slice::kRoots.w[D][j << (6 - hs)], slice::kRoots.wq[D][j << (6 - hs)], mk);
+      }
+    }
+  }
+  double y[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) y[k] = x[brev_bits(k, R)];
+#pragma unroll
+  for (int k = 0; k < N; k++) x[k] = y[k];
+}
+
+// The top digit's twiddle w^(+-low k) of row group `low`, digit value `k`,
+// from its table(s) `t` (see DecodePlan::FT).  With FT it is two factors:
+// top_first multiplies by [k << FA | ll] (mul_reg), and top_last returns
+// [k << (LO1 - FA) | lh] for the multiplication that ends the pass; without,
+// top_first does nothing and top_last returns the one table's entry.
 template <class PL>
-ZHD double top_twiddle(double x, const uint32_t* __restrict__ t, uint32_t k, uint32_t low) {
-  const TwTab tab{nullptr, t};
+ZHD double top_first(double x, const uint32_t* __restrict__ t, uint32_t k, uint32_t low, const slice::MulK& mk) {
+  if constexpr (PL::FT) {
+    const slice::Tw2 a = dec_tw(t, (k << PL::FA) | (low & ((1u << PL::FA) - 1u)));
+    return slice::mul_reg(x, a.w, a.wq, mk);
+  } else {
+    return x;
+  }
+}
+
+template <class PL>
+ZHD slice::Tw2 top_last(const uint32_t* __restrict__ t, uint32_t k, uint32_t low) {
   if constexpr (PL::FT) {
     constexpr int FA = PL::FA, HB = PL::LO1 - PL::FA;
-    const Tw a = tw_get<true>(tab, (k << FA) | (low & ((1u << FA) - 1u)));
-    const Tw b = tw_get<true>(tab, (1u << (PL::R1 + FA)) + ((k << HB) | (low >> FA)));
-    return mulmod(mulmod(x, a), b);
+    return dec_tw(t, (1u << (PL::R1 + FA)) + ((k << HB) | (low >> FA)));
   } else {
-    return mulmod(x, tw_get<true>(tab, (k << PL::LO1) | low));
+    return dec_tw(t, (k << PL::LO1) | low);
   }
 }
 
 // INTT pass on the digit of R bits at bit LO: inverse DFT over the digit,
 // then the inter-digit twiddle: the top digit's from its table(s) `tl` (see
 // DecodePlan::FT), the second digit's from the LDS table `tl` ([k << LO |
-// low]: neighbouring lanes read neighbouring words).
+// low]: neighbouring lanes read neighbouring words).  The multiplication
+// that ends the pass (times 1 for k = 0) yields the int32 LDS word directly
+// (slice::mul_mem).
 template <class PL, int LO, int R>
-ZHD void dec_intt(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ tl, const Tw* __restrict__ roots) {
+ZHD void dec_intt(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ tl, const slice::MulK& mk) {
   constexpr int RR = 1 << R;
   constexpr bool TOP = LO + R == PL::L;
   constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
   static_assert((PL::W << (PL::L - R)) % PL::NT == 0, "groups must divide evenly over the threads");
-  const TwTab own{nullptr, tl};
 #pragma unroll
   for (int gi = 0; gi < G; gi++) {
     group_fence();
@@ -110,18 +162,18 @@ ZHD void dec_intt(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
     double x[RR];
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<LO, R>(u, m), c)];
-    dft_reg<R>(x, roots + 64);
+    dec_dft<R, true>(x, mk);
     const uint32_t low = u & ((1u << LO) - 1u);
-    x[0] = reduce(x[0]);  // twiddle w^0 = 1
+    if constexpr (TOP) {
+#pragma unroll
+      for (int k = 1; k < RR; k++) x[k] = top_first<PL>(x[k], tl, (uint32_t)k, low, mk);
+    }
+    lds[lds_at<PL>(row_of<LO, R>(u, 0), c)] = slice::mul_mem(x[0], 1.0, PINV, mk);
 #pragma unroll
     for (int k = 1; k < RR; k++) {
-      if constexpr (TOP)
-        x[k] = top_twiddle<PL>(x[k], tl, (uint32_t)k, low);
-      else
-        x[k] = mulmod(x[k], tw_get<true>(own, ((uint32_t)k << LO) | low));
+      const slice::Tw2 t = TOP ? top_last<PL>(tl, (uint32_t)k, low) : dec_tw(tl, ((uint32_t)k << LO) | low);
+      lds[lds_at<PL>(row_of<LO, R>(u, k), c)] = slice::mul_mem(x[k], t, mk);
     }
-#pragma unroll
-    for (int m = 0; m < RR; m++) lds[lds_at<PL>(row_of<LO, R>(u, m), c)] = (int32_t)x[m];
   }
 }
 
@@ -132,12 +184,11 @@ ZHD void dec_intt(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
 // (z_m + z_(m+H)) (w^2)^(qm) -- each times the forward twiddle of the digit
 // above (LDS table `ta`), written to the even rows.
 template <class PL>
-ZHD void dec_mid(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ ta, const Tw* __restrict__ roots) {
+ZHD void dec_mid(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ ta, const slice::MulK& mk) {
   constexpr int R = PL::RD, RR = 1 << R, H = RR / 2, LOA = PL::LO2, RA = PL::R2;
   constexpr int G = (PL::W << (PL::L - R)) / PL::NT;
   static_assert((PL::W << (PL::L - R)) % PL::NT == 0, "groups must divide evenly over the threads");
   constexpr double FSTEP = (double)(1u << (PL::R1 + PL::R2));  // f step per m
-  const TwTab above{nullptr, ta};
 #pragma unroll
   for (int gi = 0; gi < G; gi++) {
     group_fence();
@@ -146,20 +197,19 @@ ZHD void dec_mid(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict_
     double x[RR];
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<0, R>(u, m), c)];
-    dft_reg<R>(x, roots + 64);
+    dec_dft<R, true>(x, mk);
     const double f0 = (double)((u >> PL::R2) + ((u & ((1u << PL::R2) - 1u)) << PL::R1));
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = reduce(x[m] * (f0 + m * FSTEP));
     double y[H];
 #pragma unroll
     for (int q = 0; q < H; q++) y[q] = x[q] + x[q + H];
-    dft_reg<R - 1>(y, roots);
+    dec_dft<R - 1, false>(y, mk);
 #pragma unroll
     for (int q = 0; q < H; q++) {
       const uint32_t j = row_of<0, R>(u, 2u * (uint32_t)q);
       const uint32_t k = (j >> LOA) & ((1u << RA) - 1u);
-      y[q] = mulmod(y[q], tw_get<true>(above, ((j & ((1u << LOA) - 1u)) << RA) | k));
-      lds[lds_at<PL>(j, c)] = (int32_t)y[q];
+      lds[lds_at<PL>(j, c)] = slice::mul_mem(y[q], dec_tw(ta, ((j & ((1u << LOA) - 1u)) << RA) | k), mk);
     }
   }
 }
@@ -167,7 +217,7 @@ ZHD void dec_mid(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict_
 // NTT pass on the second digit: forward DFT, forward twiddle of the top
 // digit; only the groups of even rows (the row's low bit is the group's).
 template <class PL>
-ZHD void dec_ntt2(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ ta, const Tw* __restrict__ roots) {
+ZHD void dec_ntt2(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ ta, const slice::MulK& mk) {
   constexpr int R = PL::R2, RR = 1 << R, LO = PL::LO2, LOA = PL::LO1, RA = PL::R1;
   constexpr int NG = (PL::W << (PL::L - R)) >> 1;
   static_assert(NG % PL::NT == 0 || PL::NT % NG == 0, "groups must divide evenly over the threads");
@@ -181,14 +231,13 @@ ZHD void dec_ntt2(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
     double x[RR];
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<LO, R>(u, m), c)];
-    dft_reg<R>(x, roots);
+    dec_dft<R, false>(x, mk);
 #pragma unroll
     for (int m = 0; m < RR; m++) {
       const uint32_t j = row_of<LO, R>(u, m);
-      x[m] = top_twiddle<PL>(x[m], ta, (j >> LOA) & ((1u << RA) - 1u), j & ((1u << LOA) - 1u));
+      const uint32_t kd = (j >> LOA) & ((1u << RA) - 1u), lw = j & ((1u << LOA) - 1u);
+      lds[lds_at<PL>(j, c)] = slice::mul_mem(top_first<PL>(x[m], ta, kd, lw, mk), top_last<PL>(ta, kd, lw), mk);
     }
-#pragma unroll
-    for (int m = 0; m < RR; m++) lds[lds_at<PL>(row_of<LO, R>(u, m), c)] = (int32_t)x[m];
   }
 }
 
@@ -196,12 +245,11 @@ ZHD void dec_ntt2(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
 // every value times the row's factor srow[r] (1 / (n w^r Z'(w^r)) on erased
 // rows, 0 on known ones, canonical u32), reduced to canonical u32.
 template <class PL>
-ZHD void dec_ntt1(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ srow, const Tw* __restrict__ roots) {
+ZHD void dec_ntt1(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict__ srow, const slice::MulK& mk) {
   constexpr int R = PL::R1, RR = 1 << R, LO = PL::LO1;
   constexpr int NG = (PL::W << (PL::L - R)) >> 1;
   static_assert(NG % PL::NT == 0 || PL::NT % NG == 0, "groups must divide evenly over the threads");
   constexpr int G = NG >= PL::NT ? NG / PL::NT : 1;
-  const TwTab st{nullptr, srow};
 #pragma unroll
   for (int gi = 0; gi < G; gi++) {
     group_fence();
@@ -211,11 +259,12 @@ ZHD void dec_ntt1(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
     double x[RR];
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<LO, R>(u, m), c)];
-    dft_reg<R>(x, roots);
+    dec_dft<R, false>(x, mk);
 #pragma unroll
     for (int m = 0; m < RR; m++) {
       const uint32_t row = row_of<LO, R>(u, m);
-      lds[lds_at<PL>(row, c)] = (int32_t)canonical(mulmod(x[m], tw_get<true>(st, row)));
+      const slice::Tw2 t = dec_tw(srow, row);
+      lds[lds_at<PL>(row, c)] = (int32_t)canonical(slice::mul_reg(x[m], t.w, t.wq, mk));
     }
   }
 }
@@ -288,18 +337,19 @@ ZHD void dec_load_tables(int tid, int32_t* __restrict__ lds, const DecodeTabs& T
 // Step STEP (0 .. 4) of the transform pair on the tile in LDS; steps are
 // separated by a work-group barrier on the GPU.
 template <class PL, int STEP>
-ZHD void dec_step(int tid, int32_t* __restrict__ lds, const DecodeTabs& T, const uint32_t* __restrict__ srow) {
+ZHD void dec_step(int tid, int32_t* __restrict__ lds, const DecodeTabs& T, const uint32_t* __restrict__ srow,
+                  const slice::MulK& mk) {
   const uint32_t* sb = reinterpret_cast<const uint32_t*>(lds + PL::TILE_WORDS);
   if constexpr (STEP == 0)
-    dec_intt<PL, PL::LO1, PL::R1>(tid, lds, sb + T.o_inv1, T.roots);
+    dec_intt<PL, PL::LO1, PL::R1>(tid, lds, sb + T.o_inv1, mk);
   else if constexpr (STEP == 1)
-    dec_intt<PL, PL::LO2, PL::R2>(tid, lds, sb + T.o_inv2, T.roots);
+    dec_intt<PL, PL::LO2, PL::R2>(tid, lds, sb + T.o_inv2, mk);
   else if constexpr (STEP == 2)
-    dec_mid<PL>(tid, lds, sb + T.o_fwd2, T.roots);
+    dec_mid<PL>(tid, lds, sb + T.o_fwd2, mk);
   else if constexpr (STEP == 3)
-    dec_ntt2<PL>(tid, lds, sb + T.o_fwd1, T.roots);
+    dec_ntt2<PL>(tid, lds, sb + T.o_fwd1, mk);
   else
-    dec_ntt1<PL>(tid, lds, srow, T.roots);
+    dec_ntt1<PL>(tid, lds, srow, mk);
 }
 
 // ---- host construction of the tables ------------------------------------------
@@ -379,13 +429,7 @@ DecodeTableImage build_decode_tables() {
   return im;
 }
 
-// ---- compile-time constant table (the kernel reads it from constant memory) --
-
-struct DecodeConst {
-  Tw roots[128];  // w_64^e for e < 64, then w_64^-e
-};
-
-ZHC Tw tw_const(uint32_t w) { return Tw{(double)w, (double)w / PD}; }
+// ---- plans ----------------------------------------------------------------------
 
 // Decoder plans: n = 2^L for 8 <= L <= 14, W columns so that every pass has
 // at least one group per thread; up to n = 2^12 two 512-thread work-groups
@@ -401,21 +445,8 @@ using DecPlan13 = DecodePlan<13, 1, 256, 5, 4, 4, 2>;
 using DecPlan14 = DecodePlan<14, 1, 512, 5, 5, 4, 2>;
 constexpr int DEC_FP64_MIN_LOG = 8, DEC_FP64_MAX_LOG = 14;
 
-ZHC DecodeConst make_decode_const() {
-  DecodeConst c{};
-  const uint32_t w64 = root_of_unity(6), w64i = pow_mod_host(w64, P - 2);
-  uint32_t a = 1, b = 1;
-  for (int e = 0; e < 64; e++) {
-    c.roots[e] = tw_const(a);
-    c.roots[64 + e] = tw_const(b);
-    a = mul_mod_host(a, w64);
-    b = mul_mod_host(b, w64i);
-  }
-  return c;
-}
-
-// DecodeTabs over a copy `packed` of the image and the uniform table `c`.
-inline DecodeTabs make_decode_tabs(const DecodeTableImage& im, const uint32_t* packed, const DecodeConst& c) {
+// DecodeTabs over a copy `packed` of the image.
+inline DecodeTabs make_decode_tabs(const DecodeTableImage& im, const uint32_t* packed) {
   DecodeTabs T;
   T.packed = packed;
   T.o_inv1 = im.o_inv1;
@@ -423,7 +454,6 @@ inline DecodeTabs make_decode_tabs(const DecodeTableImage& im, const uint32_t* p
   T.o_fwd2 = im.o_fwd2;
   T.o_inv2 = im.o_inv2;
   T.packed_words = (uint32_t)im.packed.size();
-  T.roots = c.roots;
   return T;
 }
 
