@@ -165,6 +165,7 @@ struct SliceSim {
   struct Bufs {
     B ibf, obf, ibn, obn, obb, obp;
   };
+  static constexpr bool kLastOddInMid = PL::EVEN_MID_MASK != 0;
 
   template <int SP>
   void step(const Bufs& b, bool front, bool back) {
@@ -172,7 +173,12 @@ struct SliceSim {
     for (int tid = 0; tid < PL::NT; tid++) {
       const Tid t(tid);
       const slice::Offs<PL> o(t, cols);
-      slice::store_odd<PL>(t, SP == 0 ? b.obp : b.obb, o, SP == 0 ? PL::STEPS - 1 : SP - 1, c4, lds.data());
+      // the previous iteration's last odd rows: here, or (plans with middle-pass
+      // copies) in middle group 1, as on the device
+      if (SP > 0)
+        slice::store_odd<PL>(t, b.obb, o, SP - 1, c4, lds.data());
+      else if (!kLastOddInMid)
+        slice::store_odd<PL>(t, b.obp, o, PL::STEPS - 1, c4, lds.data());
       if (front) slice::p1<PL>(t, lds.data(), t1w(t.w), k);
       if (back) slice::p2b<PL, SP>(t, h(tid), lds.data(), t3.data(), k);
     }
@@ -203,6 +209,7 @@ struct SliceSim {
     if constexpr (PL::even_in_mid(KB)) slice::load_step<PL>(b.ibf, o, t.w, KB, cols * 4, ev);
     slice::mid_group<PL, KB>(h(tid), k);
     if constexpr (KB > 0 && PL::even_in_mid(KB - 1)) slice::store_even<PL>(b.obf, o, t.w, KB - 1, cols * 4, ev);
+    if constexpr (KB == 1 && kLastOddInMid) slice::store_odd<PL>(t, b.obb, o, PL::STEPS - 1, cols * 4, lds.data());
     if constexpr (KB + 1 < PL::RB) mid<KB + 1>(tid, b, ev);
   }
 

@@ -193,6 +193,31 @@ def test_column_kernel_matches_row_layout(zh, torch, side, cols):
     assert np.array_equal(to_host(col[idx, :].t().contiguous()), O.encode(data, threads=4))
 
 
+def test_column_layout_on_two_streams(zh, torch):
+    # two column-major encodes in flight at once on two streams (default
+    # priority), each with its own scratch from the library's stream-ordered
+    # pool, twice in a row (the second pair reuses the pool's blocks); every
+    # value against the row-layout encode of the same data
+    shapes = [(1024, 64 * 300), (2048, 64 * 200)]
+    ins = []
+    for i, (side, cols) in enumerate(shapes):
+        t = torch.empty((side, cols), dtype=torch.int32, device="cuda")
+        zh.fill_uniform(t, seed=7 + i)
+        ins.append(t)
+    refs = [zh.encode_tensor(t, layout=0) for t in ins]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for rnd in range(2):
+        print_case("col-two-streams", round=rnd)
+        outs = [torch.empty((cols, 2 * side), dtype=torch.int32, device="cuda") for side, cols in shapes]
+        torch.cuda.synchronize()
+        for (side, cols), t, o, s in zip(shapes, ins, outs, streams):
+            zh.encode_device(t.data_ptr(), side, cols, side, o.data_ptr(), 1, s.cuda_stream)
+        torch.cuda.synchronize()
+        for o, ref in zip(outs, refs):
+            assert torch.equal(o.t().contiguous(), ref)
+
+
 def test_column_layout_under_graph_capture(zh, torch):
     # captured into a graph, the column-major encode of whole 64-column blocks
     # leaves the chunked path (its scratch comes from the stream-ordered

@@ -93,12 +93,24 @@ struct Bufs {
 // instead of after it.  The two waves that share a SIMD (w and w + 4) take
 // opposite orders, so that while one of them waits for the address unit to
 // take its loads and stores, the other keeps the SIMD busy.
+// The odd rows of a back tile's last step wait in the ODD staging through the
+// middle passes of the front tile.  Plans whose middle passes copy even rows
+// (EVM != 0) store them there, in middle group 1 (2048^2: 2.06-2.08 ms
+// against 2.09-2.10 ms, alternating runs); the others in step 0 of the next
+// iteration (at side 4096 the middle-pass store measured 0.5 % slower).
+template <unsigned EVM>
+constexpr bool kLastOddInMid = EVM != 0;
+
 template <class PL, unsigned EVM, int SP, bool MF>
 __device__ __forceinline__ void step(const Ctx<PL>& c, uint32_t c4, bool front, bool back, const Bufs<PL>& b,
                                      U4 (&ld)[PL::QI], int32_t (&held)[PL::RB][PL::RC]) {
-  // odd rows of step SP - 1 (of the previous iteration's last step for SP = 0)
+  // odd rows of step SP - 1 (of the previous iteration's last step for SP = 0,
+  // unless the middle passes stored them)
   auto odd = [&] {
-    slice::store_odd<PL>(c.t, SP == 0 ? b.obp : b.obb, c.o, SP == 0 ? PL::STEPS - 1 : SP - 1, c4, c.lds);
+    if constexpr (SP > 0)
+      slice::store_odd<PL>(c.t, b.obb, c.o, SP - 1, c4, c.lds);
+    else if constexpr (!kLastOddInMid<EVM>)
+      slice::store_odd<PL>(c.t, b.obp, c.o, PL::STEPS - 1, c4, c.lds);
   };
   // rows of step SP + 1 (the next tile's step 0 after the last step) staged,
   // then the loads of step SP + 2
@@ -150,7 +162,10 @@ __device__ __forceinline__ void step(const Ctx<PL>& c, uint32_t c4, bool front, 
 // the even output rows, so that the CU keeps memory requests in flight
 // through the passes that have no memory work of their own, and each load has
 // two groups' arithmetic to arrive before its data is needed (1 % over
-// storing in the same group).
+// storing in the same group).  With kLastOddInMid, group 1 also stores the
+// back tile's last odd rows from the ODD staging (written by the last step's
+// P1' before its barrier; the next write to ODD follows step 0's first
+// barrier, which waits for these reads).
 template <class PL, unsigned EVM, int KB>
 __device__ __forceinline__ void mid(const Ctx<PL>& c, uint32_t c4, const Bufs<PL>& b,
                                     int32_t (&held)[PL::RB][PL::RC], U4 (&ev)[PL::QI]) {
@@ -162,6 +177,10 @@ __device__ __forceinline__ void mid(const Ctx<PL>& c, uint32_t c4, const Bufs<PL
   if constexpr (STORE) {
     group_fence();
     slice::store_even<PL>(b.obf, c.o, c.wu, KB - 1, c4, ev);
+  }
+  if constexpr (KB == 1 && kLastOddInMid<EVM>) {
+    group_fence();
+    slice::store_odd<PL>(c.t, b.obb, c.o, PL::STEPS - 1, c4, c.lds);
   }
   if constexpr (KB + 1 < PL::RB) mid<PL, EVM, KB + 1>(c, c4, b, held, ev);
 }
