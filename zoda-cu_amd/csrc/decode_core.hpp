@@ -89,8 +89,8 @@ ZHD slice::Tw2 dec_tw(const uint32_t* __restrict__ t, uint32_t i) {
 // bit-reversed result renamed to natural order; |x| < B on entry gives
 // |X| < 2^R B.  Each product is slice::mul_reg's four instructions on the
 // compile-time roots.  Unlike slice::dft it does not batch the products:
-// with four waves per SIMD the other waves hide their latency, and batches
-// would need registers these kernels do not have.
+// with four waves per SIMD (two for n >= 2^13) the other waves hide their
+// latency, and batches would need registers these kernels do not have.
 template <int R, bool INV>
 ZHD void dec_dft(double (&x)[1 << R], const slice::MulK& mk) {
   constexpr int N = 1 << R;
