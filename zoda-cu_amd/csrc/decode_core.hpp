@@ -84,39 +84,6 @@ ZHD slice::Tw2 dec_tw(const uint32_t* __restrict__ t, uint32_t i) {
   return slice::Tw2{w, w * PINV};
 }
 
-// Natural-order DFT of 2^R values in place with the roots w_(2^R)^(+-1) (INV
-// selects the inverse roots), radix-2 decimation in frequency, the
-// bit-reversed result renamed to natural order; |x| < B on entry gives
-// |X| < 2^R B.  Each product is slice::mul_reg's four instructions on the
-// compile-time roots.  Unlike slice::dft it does not batch the products:
-// with four waves per SIMD (two for n >= 2^13) the other waves hide their
-// latency, and batches would need registers these kernels do not have.
-template <int R, bool INV>
-ZHD void dec_dft(double (&x)[1 << R], const slice::MulK& mk) {
-  constexpr int N = 1 << R;
-  constexpr int D = INV ? 1 : 0;
-#pragma unroll
-  for (int hs = R - 1; hs >= 0; hs--) {
-    const int half = 1 << hs;
-#pragma unroll
-    for (int b = 0; b < N; b += 2 * half) {
-#pragma unroll
-      for (int j = 0; j < half; j++) {
-        const double u = x[b + j], v = x[b + j + half];
-        x[b + j] = u + v;
-        const double d = u - v;
-        x[b + j + half] =
-            j == 0 ? d : slice::mul_reg(d, slice::kRoots.w[D][j << (6 - hs)], slice::kRoots.wq[D][j << (6 - hs)], mk);
-      }
-    }
-  }
-  double y[N];
-#pragma unroll
-  for (int k = 0; k < N; k++) y[k] = x[brev_bits(k, R)];
-#pragma unroll
-  for (int k = 0; k < N; k++) x[k] = y[k];
-}
-
 // The top digit's twiddle w^(+-low k) of row group `low`, digit value `k`,
 // from its table(s) `t` (see DecodePlan::FT).  With FT it is two factors:
 // top_first multiplies by [k << FA | ll] (mul_reg), and top_last returns
@@ -162,7 +129,7 @@ ZHD void dec_intt(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
     double x[RR];
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<LO, R>(u, m), c)];
-    dec_dft<R, true>(x, mk);
+    dft4<R, true>(x, mk);
     const uint32_t low = u & ((1u << LO) - 1u);
     if constexpr (TOP) {
 #pragma unroll
@@ -197,14 +164,14 @@ ZHD void dec_mid(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict_
     double x[RR];
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<0, R>(u, m), c)];
-    dec_dft<R, true>(x, mk);
+    dft4<R, true>(x, mk);
     const double f0 = (double)((u >> PL::R2) + ((u & ((1u << PL::R2) - 1u)) << PL::R1));
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = reduce(x[m] * (f0 + m * FSTEP));
     double y[H];
 #pragma unroll
     for (int q = 0; q < H; q++) y[q] = x[q] + x[q + H];
-    dec_dft<R - 1, false>(y, mk);
+    dft4<R - 1, false>(y, mk);
 #pragma unroll
     for (int q = 0; q < H; q++) {
       const uint32_t j = row_of<0, R>(u, 2u * (uint32_t)q);
@@ -231,7 +198,7 @@ ZHD void dec_ntt2(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
     double x[RR];
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<LO, R>(u, m), c)];
-    dec_dft<R, false>(x, mk);
+    dft4<R, false>(x, mk);
 #pragma unroll
     for (int m = 0; m < RR; m++) {
       const uint32_t j = row_of<LO, R>(u, m);
@@ -259,7 +226,7 @@ ZHD void dec_ntt1(int tid, int32_t* __restrict__ lds, const uint32_t* __restrict
     double x[RR];
 #pragma unroll
     for (int m = 0; m < RR; m++) x[m] = (double)lds[lds_at<PL>(row_of<LO, R>(u, m), c)];
-    dec_dft<R, false>(x, mk);
+    dft4<R, false>(x, mk);
 #pragma unroll
     for (int m = 0; m < RR; m++) {
       const uint32_t row = row_of<LO, R>(u, m);

@@ -43,6 +43,7 @@
 
 #include "field.hpp"
 #include "fp64.hpp"
+#include "fp64_ops.hpp"
 
 namespace zoda {
 
@@ -141,6 +142,39 @@ ZHD void dft_reg(double (&x)[1 << R], const Tw* __restrict__ rt) {
           const Tw t = rt[j << (5 - hs)];  // w_(2 half)^j = w_64^(j * 64 / (2 half))
           x[b + j + half] = mulmod(d, t.w, t.wq);
         }
+      }
+    }
+  }
+  double y[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) y[k] = x[brev_bits(k, R)];
+#pragma unroll
+  for (int k = 0; k < N; k++) x[k] = y[k];
+}
+
+// The same DFT with the four-instruction products (slice::mul_reg) on the
+// compile-time roots w_128^(+-e) (INV selects the inverse roots), unbatched:
+// with several waves per SIMD the other waves hide each product's latency,
+// and batches (slice::dft) need registers the decoder's kernels do not have.
+// (The per-side encode kernels keep dft_reg: with the product's constants
+// held in registers through a pass, their plans for sides 256 to 4096 grew
+// by up to 60 registers a thread or spilled.)
+template <int R, bool INV>
+ZHD void dft4(double (&x)[1 << R], const slice::MulK& mk) {
+  constexpr int N = 1 << R;
+  constexpr int D = INV ? 1 : 0;
+#pragma unroll
+  for (int hs = R - 1; hs >= 0; hs--) {
+    const int half = 1 << hs;
+#pragma unroll
+    for (int b = 0; b < N; b += 2 * half) {
+#pragma unroll
+      for (int j = 0; j < half; j++) {
+        const double u = x[b + j], v = x[b + j + half];
+        x[b + j] = u + v;
+        const double d = u - v;
+        x[b + j + half] =
+            j == 0 ? d : slice::mul_reg(d, slice::kRoots.w[D][j << (6 - hs)], slice::kRoots.wq[D][j << (6 - hs)], mk);
       }
     }
   }
