@@ -321,7 +321,7 @@ __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_chunks(DecChunks a, cons
       const bool live = b < ng;
       const uint32_t r = live ? a.rows[(b % KB) * DB + tid] : 0u;
       rid[buf][tid] = r;
-      zr[buf][tid] = live ? a.zk[r] : 0u;
+      zr[buf][tid] = live ? mont_mul(a.zk[r], 1u) : 0u;  // Z(w^r) out of Montgomery form
     }
   };
   auto fetch = [&](uint32_t b, int buf) {  // after a barrier that follows stage(b, buf)
@@ -355,14 +355,18 @@ __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_chunks(DecChunks a, cons
       for (int g = 0; g < GI; g++) {
         const int idx = tid + g * PL::NT;
         if (idx >= DB * DB / 4) break;
-        const uint32_t q = idx / 16, c4 = 4 * (idx % 16), r = rid[buf][q], z = zr[buf][q];
+        const uint32_t q = idx / 16, c4 = 4 * (idx % 16), r = rid[buf][q];
         const U4 e = gv[g];
         if (!(r & 1u)) store16(a.data + (size_t)(r >> 1) * a.cols + a.gc0 + cb + c4, e);  // a known data row
+        // y_r = value * Z(w^r) as the int32 the column kernel's LDS takes (|y| < 2^30,
+        // not canonical): the FP64 product, not Montgomery's four 32-bit
+        // multiplies at a quarter of the rate
+        const double zw = (double)zr[buf][q], zq = zw * PINV;
         uint32_t* d = sh + q * DB_PITCH + c4;
-        d[0] = mont_mul(e.x, z);
-        d[1] = mont_mul(e.y, z);
-        d[2] = mont_mul(e.z, z);
-        d[3] = mont_mul(e.w, z);
+        d[0] = (uint32_t)slice::mul_mem((double)e.x, zw, zq, mk);
+        d[1] = (uint32_t)slice::mul_mem((double)e.y, zw, zq, mk);
+        d[2] = (uint32_t)slice::mul_mem((double)e.z, zw, zq, mk);
+        d[3] = (uint32_t)slice::mul_mem((double)e.w, zw, zq, mk);
       }
       __syncthreads();
       for (int idx = tid; idx < DB * DB / 4; idx += PL::NT) {
