@@ -359,8 +359,9 @@ __global__ __launch_bounds__(PL::NT, PL::MINW) void dec_chunks(DecChunks a, cons
         const U4 e = gv[g];
         if (!(r & 1u)) store16(a.data + (size_t)(r >> 1) * a.cols + a.gc0 + cb + c4, e);  // a known data row
         // y_r = value * Z(w^r) as the int32 the column kernel's LDS takes (|y| < 2^30,
-        // not canonical): the FP64 product, not Montgomery's four 32-bit
-        // multiplies at a quarter of the rate
+        // not canonical): the four-instruction FP64 product in place of a
+        // Montgomery product (four 32-bit multiplies and the 64-bit additions
+        // and the final correction around them)
         const double zw = (double)zr[buf][q], zq = zw * PINV;
         uint32_t* d = sh + q * DB_PITCH + c4;
         d[0] = (uint32_t)slice::mul_mem((double)e.x, zw, zq, mk);
