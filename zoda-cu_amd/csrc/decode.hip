@@ -68,6 +68,28 @@ __device__ uint32_t mont_pow(uint32_t a, uint32_t e) {
   return r;
 }
 
+// prod_j (x - ep_m[j]) in Montgomery form over the ne erased points (SKIP:
+// leaving out a point equal to x).  Sixteen independent partial products: one
+// chain of ne dependent Montgomery products per thread left this kernel
+// latency-bound (164 us at n = 4096 with 2048 erasures).
+template <bool SKIP>
+__device__ uint32_t prod_diffs(uint32_t x, const uint32_t* __restrict__ ep_m, uint32_t ne) {
+  constexpr int NA = 16;
+  uint32_t a[NA];
+#pragma unroll
+  for (int q = 0; q < NA; q++) a[q] = MONT_ONE;
+  auto term = [&](uint32_t e) { return (SKIP && e == x) ? MONT_ONE : sub_mod(x, e); };
+  uint32_t j = 0;
+  for (; j + NA <= ne; j += NA) {
+#pragma unroll
+    for (int q = 0; q < NA; q++) a[q] = mont_mul(a[q], term(ep_m[j + q]));
+  }
+  for (; j < ne; j++) a[0] = mont_mul(a[0], term(ep_m[j]));
+#pragma unroll
+  for (int q = 1; q < NA; q++) a[0] = mont_mul(a[0], a[q]);
+  return a[0];
+}
+
 // x_m[r] = w^r in Montgomery form; ep_m: the erased points, Montgomery form.
 // zk_m[r] = Z(w^r) (0 on erased rows).  FP64 path (`deriv`): srow[r] =
 // 1 / (n w^r Z'(w^r)) on the erased even rows, Z'(w^e) = prod over the other
@@ -85,9 +107,7 @@ __global__ void dec_prep(const uint32_t* __restrict__ x_m, const uint8_t* __rest
       zk_m[t] = 0;
       return;
     }
-    uint32_t acc = MONT_ONE;
-    for (uint32_t j = 0; j < ne; j++) acc = mont_mul(acc, sub_mod(x_m[t], ep_m[j]));
-    zk_m[t] = acc;
+    zk_m[t] = prod_diffs<false>(x_m[t], ep_m, ne);
     return;
   }
   if (deriv) {
@@ -97,17 +117,13 @@ __global__ void dec_prep(const uint32_t* __restrict__ x_m, const uint8_t* __rest
       return;
     }
     const uint32_t x = x_m[r];
-    uint32_t acc = mont_mul(n_m, x);  // n w^r
-    for (uint32_t j = 0; j < ne; j++)
-      if (ep_m[j] != x) acc = mont_mul(acc, sub_mod(x, ep_m[j]));
+    const uint32_t acc = mont_mul(mont_mul(n_m, x), prod_diffs<true>(x, ep_m, ne));  // n w^r Z'(w^r)
     srow[r] = mont_mul(mont_pow(acc, P - 2), 1u);  // mont_mul by 1 leaves Montgomery form
     return;
   }
   const uint32_t r = brev_n(t - n, log_n);
   const uint32_t x = mont_mul(h_m, x_m[r]);
-  uint32_t acc = MONT_ONE;
-  for (uint32_t j = 0; j < ne; j++) acc = mont_mul(acc, sub_mod(x, ep_m[j]));
-  srow[t - n] = mont_pow(acc, P - 2);
+  srow[t - n] = mont_pow(prod_diffs<false>(x, ep_m, ne), P - 2);
 }
 
 // T[c * n + r] = enc[r * cols + c] * Z(w^r) (a 32 x 32 tile per block).  The
