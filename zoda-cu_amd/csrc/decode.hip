@@ -639,7 +639,9 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
   const size_t words = up_words + n + n;
   uint32_t* d_blk = nullptr;
   uint32_t* T = nullptr;
-  ZODA_HIP(hipMalloc((void**)&d_blk, words * 4 + n));
+  // stream-ordered, from the library's pool: no device-wide synchronisation
+  // for the allocation or its release
+  ZODA_TRY(pool_alloc(st, words * 4 + n, s, (void**)&d_blk));
   uint32_t* d_rows = d_blk;
   uint32_t* d_x = d_rows + n;
   uint32_t* d_hpow = d_x + n;
@@ -656,8 +658,10 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
   host.insert(host.end(), hinv_m.begin(), hinv_m.end());
   host.insert(host.end(), ep_m.begin(), ep_m.end());
   int rc = 0;
-  hip_note(rc, hipMemcpy(d_blk, host.data(), up_words * 4, hipMemcpyHostToDevice), "decode vectors upload");
-  if (!rc) hip_note(rc, hipMemcpy(d_known, known.data(), n, hipMemcpyHostToDevice), "decode vectors upload");
+  // queued on the stream (the host vectors live until the synchronisation below)
+  hip_note(rc, hipMemcpyAsync(d_blk, host.data(), up_words * 4, hipMemcpyHostToDevice, s), "decode vectors upload");
+  if (!rc)
+    hip_note(rc, hipMemcpyAsync(d_known, known.data(), n, hipMemcpyHostToDevice, s), "decode vectors upload");
   // Whole 64-column blocks of 16-byte aligned squares take the chunked FP64
   // path (dec_chunks, its own scratch); otherwise the scratch holds the k
   // known rows per column (FP64 path) or all n rows.
@@ -703,8 +707,7 @@ extern "C" int zoda_hip_decode_device(const uint32_t* d_enc, uint32_t k, uint32_
     }
   }
   if (T) hip_note(rc, hipFreeAsync(T, s), "hipFreeAsync");
-  // d_blk is read by the kernels just queued: free it once they are done.
+  hip_note(rc, hipFreeAsync(d_blk, s), "hipFreeAsync");  // after the kernels that read it
   hip_note(rc, hipStreamSynchronize(s), "decode");
-  (void)hipFree(d_blk);
   return rc;
 }
